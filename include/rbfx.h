@@ -1,0 +1,182 @@
+/*
+ * rbfx.h - C ABI of the MI355X (gfx950) RBF-FD hot path.
+ *
+ * Plain C: pointers and sizes only. Every function returns RBFX_OK (0) or a negative
+ * RBFX_E_* code and never exits the process; rbfx_last_error() gives the message.
+ * A context owns one HIP stream and all device buffers it creates; it is single-threaded
+ * (like one rank of the reference) and asynchronous on its stream.
+ *
+ * Each entry point names the reference interface it replaces (paths relative to the
+ * bollig/rbffd_gpu tree).
+ */
+#ifndef RBFX_H
+#define RBFX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------------------- */
+#define RBFX_OK 0
+#define RBFX_E_INVALID -1      /* bad argument (null, size, mask, unsupported option)   */
+#define RBFX_E_HIP -2          /* a HIP runtime call failed                              */
+#define RBFX_E_NOMEM -3        /* device or host allocation failed                       */
+#define RBFX_E_NODEVICE -4     /* no usable GPU                                          */
+#define RBFX_E_SINGULAR -5     /* at least one stencil system had a zero pivot           */
+#define RBFX_E_STATE -6        /* call out of order (e.g. step before state was set)     */
+#define RBFX_E_TOO_LARGE -7    /* size beyond a documented limit                         */
+
+/* ---- derivative types: bit values of RBFFD::DerType (src/rbffd/rbffd.h:43-60) -------- */
+#define RBFX_X 0x1u
+#define RBFX_Y 0x2u
+#define RBFX_Z 0x4u
+#define RBFX_LAPL 0x8u
+#define RBFX_R 0x10u
+#define RBFX_HV 0x20u
+#define RBFX_LAMBDA 0x40u
+#define RBFX_THETA 0x80u
+#define RBFX_LSFC 0x100u
+#define RBFX_XSFC 0x200u
+#define RBFX_YSFC 0x400u
+#define RBFX_ZSFC 0x800u
+#define RBFX_INTERP 0x8000u
+/* ALT_XSFC/ALT_YSFC/ALT_ZSFC (0x1000-0x4000) are linear combinations of X,Y,Z weights
+ * (src/rbffd/rbffd.cpp:337-347); they are formed on the host side, not solved for. */
+
+/* ---- precision selectors ------------------------------------------------------------- */
+#define RBFX_F64 0
+#define RBFX_F32 1
+
+/* weight-solve modes */
+#define RBFX_SOLVE_F64_LU 0          /* FP64 partial-pivot LU in LDS                       */
+#define RBFX_SOLVE_F32_LU_F64_IR 1   /* FP32 LU + FP64-residual refinement, FP64 fallback  */
+
+/* per-stencil diagnostic path flags (diag_out[4*s+3]) */
+#define RBFX_PATH_F64 0
+#define RBFX_PATH_F32_IR 1
+#define RBFX_PATH_F32_FALLBACK_F64 2
+#define RBFX_PATH_SINGULAR 3
+
+typedef struct rbfx_ctx rbfx_ctx;
+typedef struct rbfx_ell rbfx_ell;
+typedef struct rbfx_rk4 rbfx_rk4;
+
+/* ---- context ------------------------------------------------------------------------- */
+/* Replaces the OpenCL context/queue set-up of CLBaseClass (src/utils/opencl/cl_base_class.h:14-48). */
+int rbfx_create(int device_ordinal, rbfx_ctx** out);
+void rbfx_destroy(rbfx_ctx* ctx);
+/* Message of the last failure on this context (or of rbfx_create when ctx is NULL). */
+const char* rbfx_last_error(const rbfx_ctx* ctx);
+/* Blocks until all work queued on the context's stream is done (queue.finish()). */
+int rbfx_sync(rbfx_ctx* ctx);
+/* Device name and peak figures as reported by the HIP runtime. */
+int rbfx_device_info(rbfx_ctx* ctx, char* name, int32_t name_len, int32_t* n_cu, int32_t* clock_khz);
+
+/* ---- device memory helpers (the ctx's stream; host pointers are pageable) ------------ */
+int rbfx_dev_alloc(rbfx_ctx* ctx, uint64_t bytes, void** d_ptr);
+int rbfx_dev_free(rbfx_ctx* ctx, void* d_ptr);
+int rbfx_h2d(rbfx_ctx* ctx, void* d_dst, const void* h_src, uint64_t bytes);
+int rbfx_d2h(rbfx_ctx* ctx, void* h_dst, const void* d_src, uint64_t bytes);
+int rbfx_d2d(rbfx_ctx* ctx, void* d_dst, const void* d_src, uint64_t bytes);
+
+/* Stream-ordered timer: records a HIP event pair on the ctx stream around the work queued
+ * between start and stop; stop waits for completion and returns elapsed milliseconds. */
+int rbfx_timer_start(rbfx_ctx* ctx);
+int rbfx_timer_stop(rbfx_ctx* ctx, double* ms);
+
+/* ---- host-side stencil generation ---------------------------------------------------- */
+/* Grid::generateStencilsKDTree (src/grids/grid_interface.cpp:688-725) with the reference's
+ * kd-tree (external/kdtree/KDTree.h): out_idx[i*k + j] = j-th nearest node of node i
+ * (j = 0 is node i itself), bit-identical to the reference, ties included.
+ * Host memory, CPU work; n_threads <= 0 uses all OpenMP threads. */
+int rbfx_knn_kdtree(int32_t n_nodes, const double* xyz, int32_t k, int32_t* out_idx, int32_t n_threads);
+/* Seeded node set on the unit sphere: golden-angle (Fibonacci) spiral plus an independent
+ * uniform +-jitter per coordinate (SplitMix64 counter stream), not re-projected.
+ * Stands in for GridReader::readNodeList's input files (src/grids/grid_reader.cpp:36-100). */
+int rbfx_nodes_fibonacci(int32_t n_nodes, double jitter, uint64_t seed, double* xyz);
+
+/* ---- batched weight computation ------------------------------------------------------ */
+/* RBFFD::computeAllWeightsForAllStencils -> computeAllWeightsForStencil_Direct
+ * (src/rbffd/rbffd.cpp:90-103, 225-352) with getStencilLHS/distanceMatrix (129-191, 888-969)
+ * and getStencilMultiRHS/getStencilRHS (106-125, 356-537), for all stencils in one call.
+ *
+ * Stencils are CSR-shaped and may have uneven sizes: stencil s uses nodes
+ * d_st_idx[d_st_off[s] .. d_st_off[s+1]), first entry = centre. All pointers are device
+ * pointers: d_xyz N*3 float64, d_eps one shape parameter per stencil.
+ * dertypes: RBFX_* mask; one output column per set bit in ascending bit order.
+ * dim: dimension of LAPL and of the HV recurrence (RBFFD(..., dim_num, ...)).
+ * poly_deg: -1 = no polynomial, 0 = the reference's single constant monomial (ONE_MONOMIAL,
+ *   rbffd.cpp:1), 1..3 = full monomial basis in centred coordinates (x - x_centre).
+ * hv_k, hv_scale: Delta^k order and the column scale -gamma*N^-k (RBFFD::getHVScalar,
+ *   rbffd.h:227-238); hv_scale multiplies the HV column only.
+ * Output (device): d_w[t * nnz + d_st_off[s] + j] for type column t, node j < n_s
+ *   (the polynomial multipliers are not stored). d_diag (optional, may be NULL):
+ *   4 doubles per stencil {backward error ||R - M W||_max/(||M||_max ||W||_max), max |pivot|
+ *   ratio, refinement iterations, path flag}.
+ * n_singular (optional host pointer) receives the count of stencils with a zero pivot;
+ * their weights are left as NaN and the call returns RBFX_E_SINGULAR. */
+int rbfx_weights(rbfx_ctx* ctx, int32_t n_stencils, const int32_t* d_st_off, const int32_t* d_st_idx,
+                 int32_t n_nodes, const double* d_xyz, const double* d_eps, uint32_t dertypes, int32_t dim,
+                 int32_t poly_deg, int32_t hv_k, double hv_scale, int32_t mode, double* d_w, double* d_diag,
+                 int32_t* n_singular);
+
+/* ---- ELL operators (fixed stencil width) --------------------------------------------- */
+/* Builds the device layout of n_ops operators that share one index array, the analogue of
+ * RBFFD_CL::updateStencilsOnGPU/updateWeightsOnGPU (src/rbffd/rbffd_cl.cpp:215-257, 306-463)
+ * and RBFFD_VCL's ell_matrix (src/rbffd/rbffd_vcl.cpp:224-299).
+ * Inputs (device): d_idx n_rows*width int32 row-major (row i, slot j; pad slots = centre),
+ * d_w[o] n_rows*width float64 row-major (pad weight 0); storage RBFX_F64 or RBFX_F32.
+ * dertype_bits[o] (host) labels operator o. */
+int rbfx_ell_create(rbfx_ctx* ctx, int32_t n_rows, int32_t n_cols, int32_t width, const int32_t* d_idx,
+                    int32_t n_ops, const uint32_t* dertype_bits, const double* const* d_w, int32_t storage,
+                    rbfx_ell** out);
+void rbfx_ell_destroy(rbfx_ell* ell);
+int rbfx_ell_info(const rbfx_ell* ell, int32_t* n_rows, int32_t* width, int32_t* n_ops, int32_t* storage);
+
+/* Derivative application RBFFD::applyWeightsForDeriv (src/rbffd/rbffd.cpp:669-688,
+ * rbffd_cl.cpp:526-694): out[i] = sum_j W_op[i][j] * u[idx[i][j]] for operator index op.
+ * d_u: n_cols values, d_out: n_rows values, both of type field_type (RBFX_F64/F32);
+ * accumulation in accum (RBFX_F64/F32). */
+int rbfx_spmv(rbfx_ell* ell, int32_t op, const void* d_u, void* d_out, int32_t field_type, int32_t accum);
+
+/* Fused multi-operator, multi-field application:
+ * out[o][i][f] = sum_j W_o[i][j] * u[idx[i][j]][f] for every operator o of the set and
+ * field f < n_fields; u is node-major (n_cols x n_fields), out is n_ops x n_rows x n_fields. */
+int rbfx_spmm(rbfx_ell* ell, int32_t n_fields, const void* d_u, void* d_out, int32_t field_type, int32_t accum);
+
+/* ---- RK4 time stepping ---------------------------------------------------------------- */
+/* TimeDependentPDE::advanceRungeKutta4 (src/pdes/time_dependent_pde.cpp:172-304), the OpenCL
+ * evaluateRK4_* / advanceRK4_classic kernels (src/cl_kernels/rk4_thread_per_stencil.cl:13-138)
+ * and TimeDependentPDE_VCL::advanceRK4 (src/pdes/time_dependent_pde_vcl.cpp:246-393), with
+ * the PDE right-hand side as a per-node linear combination of the operators:
+ *     f[i][q] = sum_o c_o[i] * (D_o u)[i][q]
+ * where c_o is a device array of n_rows float64 or NULL for the constant 1. The cosine bell
+ * (src/pdes/hyperbolic/cosine_bell.cpp:23-70) is c_LAMBDA = -u_vel/a, c_THETA = -v_vel/a,
+ * c_HV = 1. Each step runs 4 fused stage kernels (gather-SpMM + right-hand side + stage
+ * update) captured once into a hipGraph. State u is float64, node-major n_rows x n_fields. */
+typedef struct rbfx_rk4_desc {
+    rbfx_ell* ell;             /* operators (n_rows == n_cols)                              */
+    int32_t n_fields;          /* fields advanced together                                  */
+    int32_t field_storage;     /* RBFX_F64: stage fields in FP64; RBFX_F32: stored in FP32   */
+    int32_t accum;             /* RBFX_F64 or RBFX_F32 accumulation of the gathers          */
+    const double* d_coef[16];  /* per-operator coefficient arrays (NULL = 1), n_ops used    */
+    int32_t use_graph;         /* 1: capture a step into a hipGraph and replay it           */
+} rbfx_rk4_desc;
+
+int rbfx_rk4_create(rbfx_ctx* ctx, const rbfx_rk4_desc* desc, rbfx_rk4** out);
+void rbfx_rk4_destroy(rbfx_rk4* rk);
+/* TimeDependentPDE_HIP::fillInitialConditions / syncCPUtoGPU analogues (device pointers). */
+int rbfx_rk4_set_state(rbfx_rk4* rk, const double* d_u);
+int rbfx_rk4_get_state(rbfx_rk4* rk, double* d_u);
+/* Advances n_steps steps of size dt (TimeDependentPDE::advance(RK4, dt) repeated). */
+int rbfx_rk4_step(rbfx_rk4* rk, double dt, int32_t n_steps);
+/* One stage kernel launch (stage 1..4) for profiling the dominant kernel in isolation. */
+int rbfx_rk4_stage(rbfx_rk4* rk, int32_t stage, double dt);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RBFX_H */

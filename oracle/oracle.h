@@ -1,0 +1,77 @@
+/*
+ * oracle.h - CPU restatement of bollig/rbffd_gpu's hot path, used ONLY as test
+ * infrastructure: by tests/, by bench.py's cpu_baseline leg and by the smoke check, as the
+ * checker. Nothing in the product (rbffd_gpu_amd/, include/) links or calls it.
+ *
+ * Each function names the reference file:line it restates. Layouts match the product's
+ * C ABI so results compare directly:
+ *   stencils  CSR: st_off[n_st+1], st_idx[nnz] (first entry of each stencil = centre)
+ *   weights   w[t*nnz + st_off[s] + j], t = index of the t-th set DerType bit (ascending)
+ *   fields    node-major u[node*n_fields + f]
+ *
+ * Pinning status of each function is recorded in the header comment of oracle.cpp.
+ */
+#ifndef RBFX_ORACLE_H
+#define RBFX_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Gaussian RBF primitives (src/rbffd/rbfs/rbf_gaussian.h), evaluated at diff = (dx,dy,dz). */
+double orc_rbf_eval(double eps, double dx, double dy, double dz);
+double orc_rbf_xderiv(double eps, double dx, double dy, double dz);
+double orc_rbf_lapl(double eps, int dim, double dx, double dy, double dz);
+double orc_rbf_rderiv_over_r(double eps, double dx, double dy, double dz);
+double orc_rbf_rderiv2(double eps, double dx, double dy, double dz);
+double orc_rbf_hv(double eps, int dim, int k, double dx, double dy, double dz);
+double orc_hv_pk(double eps, int dim, int k, double r2);
+
+/* Number of polynomial terms for poly_deg in {-1,0,1,2,3}. */
+int orc_num_monomials(int poly_deg);
+
+/* Builds the saddle-point system of one stencil (row-major m x m LHS, m x T RHS). */
+int orc_stencil_system(const double* xyz, const int32_t* st, int32_t n, double eps, uint32_t dertypes,
+                       int32_t dim, int32_t poly_deg, int32_t hv_k, double* M, double* R);
+
+/* In-place partial-pivoting LU solve of M X = R (row-major, m x m and m x nrhs). Returns 0,
+ * or k+1 when the k-th pivot is exactly zero. */
+int orc_lu_solve(int32_t m, int32_t nrhs, double* M, double* R);
+
+/* All weights (RBFFD::computeAllWeightsForAllStencils). diag may be NULL; else 2 doubles per
+ * stencil {normwise backward error, LU pivot growth}. Returns number of singular stencils. */
+int orc_weights(int32_t n_st, const int32_t* st_off, const int32_t* st_idx, const double* xyz,
+                const double* eps, uint32_t dertypes, int32_t dim, int32_t poly_deg, int32_t hv_k,
+                double hv_scale, double* w, double* diag, int32_t n_threads);
+
+/* Derivative application (RBFFD::applyWeightsForDeriv CPU loop). */
+void orc_apply(int32_t n_rows, const int32_t* st_off, const int32_t* st_idx, const double* w,
+               const double* u, double* out);
+
+/* Multi-operator, multi-field application with fixed width n (row-major idx/w per op):
+ * out[o*n_rows*F + i*F + f] = sum_j w[o][i*n+j] * u[idx[i*n+j]*F + f]. */
+void orc_spmm(int32_t n_rows, int32_t n, const int32_t* idx, int32_t n_ops, const double* w, int32_t n_fields,
+              const double* u, double* out, int32_t n_threads);
+
+/* One classic RK4 step (TimeDependentPDE::advanceRungeKutta4) of du/dt = f(u) with
+ * f[i][q] = sum_o c_o[i] (D_o u)[i][q] (c_o NULL -> 1); u updated in place. */
+void orc_rk4_step(int32_t n_rows, int32_t n, const int32_t* idx, int32_t n_ops, const double* w,
+                  const double* const* coef, int32_t n_fields, double dt, double* u, int32_t n_threads);
+
+/* Cosine bell (src/pdes/hyperbolic/cosine_bell.{h,cpp}, exact_advection.h). */
+void orc_cart2sph(double x, double y, double z, double* theta, double* phi, double* r);
+void orc_cosine_bell_coefs(int32_t n, const double* xyz, double a, double alpha, double t_rev,
+                           double* c_lambda, double* c_theta);
+double orc_cosine_bell_exact(double x, double y, double z, double a);
+void orc_cosine_bell_rhs(int32_t n_rows, const int32_t* st_off, const int32_t* st_idx, const double* w_lambda,
+                         const double* w_theta, const double* w_hv, const double* xyz, double a, double alpha,
+                         double t_rev, int32_t use_hv, const double* u, double* f);
+
+/* PDE::calcSolNorms (src/pdes/pde.cpp:463-533): out = {l1abs, l1rel, l2abs, l2rel, liabs, lirel}. */
+void orc_sol_norms(int32_t n, const double* approx, const double* exact, double* out6);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,307 @@
+// Host side of stencil generation: seeded node sets and kd-tree k-nearest-neighbour stencils.
+//
+// The kd-tree follows the reference's algorithm step for step so that stencil index lists
+// match it bit for bit, ties included:
+//   * tree build: per-dimension heap sort of coordinates (external/kdtree/KDTree.h:179-202,
+//     MinHeap push/pop/heapsort external/kdtree/MyHeaps.h:268-455), then recursive median
+//     split with the median going left, iMedian = floor((numel-1)/2), children visited in
+//     L-then-R order and a stable re-partition of every dimension's sorted list
+//     (KDTree.h:267-330);
+//   * query: Friedman-Bentley-Finkel search (KDTree.h:459-505) with a bounded max-heap whose
+//     replacement rule is "top > d" (KDTree.h:470) and bounds test KDTree.h:566-590; results
+//     are popped from the heap and reversed (KDTree.h:407-433);
+//   * distance: d = sum over d=0,1,2 of (a_d-b_d)^2 in that order (KDTree.h:46-52), built
+//     with -ffp-contract=off so no fused multiply-add changes the rounding;
+//   * stencil: row = the k returned ids, no radius trimming (grid_interface.cpp:688-725),
+//     k clamped to N (grid_interface.cpp:968-1002).
+// The heaps are re-implemented here (the ordering of equal keys depends on their exact
+// sift rules, which is why those rules are kept). The tree itself is stored flat with
+// in-place stable partitions instead of the reference's per-node vector copies; that changes
+// memory use, not the resulting tree. Queries run in parallel (OpenMP), each with its own heap.
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "rbfx_host_internal.h"
+
+namespace {
+
+struct KeyIdx {
+    double key;
+    int idx;
+};
+
+// Min-heap with the reference's sift rules: push appends then swaps with the parent while
+// parent.key > cur.key; pop moves the tail to the root and sifts down choosing the strictly
+// smaller child (left tested first).
+class MinHeapD {
+  public:
+    explicit MinHeapD(size_t cap) { h_.reserve(cap); }
+    void push(double key, int idx) {
+        h_.push_back({key, idx});
+        size_t cur = h_.size() - 1;
+        while (cur > 0) {
+            size_t par = (cur - 1) >> 1;
+            if (h_[par].key > h_[cur].key) {
+                std::swap(h_[par], h_[cur]);
+                cur = par;
+            } else {
+                break;
+            }
+        }
+    }
+    bool empty() const { return h_.empty(); }
+    KeyIdx pop() {
+        KeyIdx top = h_[0];
+        h_[0] = h_.back();
+        h_.pop_back();
+        size_t cur = 0, n = h_.size();
+        for (;;) {
+            size_t l = 2 * cur + 1, r = 2 * cur + 2, s;
+            s = (l < n && h_[l].key < h_[cur].key) ? l : cur;
+            if (r < n && h_[r].key < h_[s].key) s = r;
+            if (s == cur) break;
+            std::swap(h_[cur], h_[s]);
+            cur = s;
+        }
+        return top;
+    }
+
+  private:
+    std::vector<KeyIdx> h_;
+};
+
+// Bounded max-heap for the k best candidates, same sift rules mirrored (strict comparisons).
+class MaxHeapD {
+  public:
+    void reset(int cap) {
+        n_ = 0;
+        if ((int)h_.size() < cap + 1) h_.resize(cap + 1);
+    }
+    int size() const { return n_; }
+    const KeyIdx& top() const { return h_[0]; }
+    void push(double key, int idx) {
+        int cur = n_++;
+        h_[cur] = {key, idx};
+        while (cur > 0) {
+            int par = (cur - 1) >> 1;
+            if (h_[par].key < h_[cur].key) {
+                std::swap(h_[par], h_[cur]);
+                cur = par;
+            } else {
+                break;
+            }
+        }
+    }
+    void pop() {
+        h_[0] = h_[n_ - 1];
+        n_--;
+        int cur = 0;
+        for (;;) {
+            int l = 2 * cur + 1, r = 2 * cur + 2, g;
+            g = (l < n_ && h_[l].key > h_[cur].key) ? l : cur;
+            if (r < n_ && h_[r].key > h_[g].key) g = r;
+            if (g == cur) break;
+            std::swap(h_[cur], h_[g]);
+            cur = g;
+        }
+    }
+
+  private:
+    std::vector<KeyIdx> h_;
+    int n_ = 0;
+};
+
+struct TreeNode {
+    double key;
+    int left, right;  // -1 when absent
+    int pidx;         // >= 0 for a leaf
+};
+
+class KdTree3 {
+  public:
+    KdTree3(const double* xyz, int n) : xyz_(xyz), n_(n) {
+        // sorter[d]: point ids in ascending coordinate d, by heap sort
+        for (int d = 0; d < 3; d++) {
+            sorter_[d].resize(n);
+            MinHeapD heap(n);
+            for (int p = 0; p < n; p++) heap.push(xyz[3 * p + d], p);
+            int i = 0;
+            while (!heap.empty()) sorter_[d][i++] = heap.pop().idx;
+        }
+        side_.assign(n, 0);
+        tmp_.resize(n);
+        nodes_.reserve(2 * (size_t)n);
+        build(0, n, 0);
+        for (int d = 0; d < 3; d++) std::vector<int>().swap(sorter_[d]);
+        std::vector<int>().swap(tmp_);
+        std::vector<char>().swap(side_);
+    }
+
+    // k nearest of point q (ids written closest first).
+    void knn(const double* q, int k, MaxHeapD& pq, int* out) const {
+        double bmin[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+        double bmax[3] = {DBL_MAX, DBL_MAX, DBL_MAX};
+        pq.reset(k);
+        search(q, k, 0, 0, bmin, bmax, pq);
+        int cnt = pq.size();
+        // pop all: farthest first; keep them and reverse
+        for (int i = cnt - 1; i >= 0; i--) {
+            out[i] = pq.top().idx;
+            pq.pop();
+        }
+    }
+
+  private:
+    int build(int lo, int hi, int dim) {
+        int numel = hi - lo;
+        int node_idx = (int)nodes_.size();
+        nodes_.push_back({0.0, -1, -1, -1});
+        if (numel == 1) {
+            nodes_[node_idx].pidx = sorter_[dim][lo];
+            nodes_[node_idx].key = 0.0;
+            return node_idx;
+        }
+        int imed = (int)std::floor((numel - 1) / 2.0);
+        int pmed = sorter_[dim][lo + imed];
+        int nl = imed + 1;
+        for (int i = lo; i < hi; i++) side_[sorter_[dim][i]] = (i - lo <= imed) ? 'l' : 'r';
+        for (int d = 0; d < 3; d++) {
+            // stable partition of sorter_[d][lo,hi) into l then r
+            int il = lo, ir = 0;
+            for (int i = lo; i < hi; i++) {
+                int p = sorter_[d][i];
+                if (side_[p] == 'l') sorter_[d][il++] = p;
+                else tmp_[ir++] = p;
+            }
+            std::memcpy(&sorter_[d][lo + nl], tmp_.data(), sizeof(int) * (size_t)ir);
+        }
+        double key = xyz_[3 * pmed + dim];
+        int l = build(lo, lo + nl, (dim + 1) % 3);
+        int r = build(lo + nl, hi, (dim + 1) % 3);
+        nodes_[node_idx].key = key;
+        nodes_[node_idx].left = l;
+        nodes_[node_idx].right = r;
+        nodes_[node_idx].pidx = -1;
+        return node_idx;
+    }
+
+    static bool overlap(const double* q, int k, const double* bmin, const double* bmax, const MaxHeapD& pq) {
+        if (pq.size() < k) return true;
+        double sum = 0.0;
+        double best = pq.top().key;
+        for (int d = 0; d < 3; d++) {
+            if (q[d] < bmin[d]) {
+                sum += (q[d] - bmin[d]) * (q[d] - bmin[d]);
+                if (sum > best) return false;
+            } else if (q[d] > bmax[d]) {
+                sum += (q[d] - bmax[d]) * (q[d] - bmax[d]);
+                if (sum > best) return false;
+            }
+        }
+        return true;
+    }
+
+    void search(const double* q, int k, int node_idx, int dim, double* bmin, double* bmax, MaxHeapD& pq) const {
+        const TreeNode& nd = nodes_[node_idx];
+        if (nd.pidx >= 0) {
+            const double* p = xyz_ + 3 * (size_t)nd.pidx;
+            double dist = 0.0;
+            for (int i = 0; i < 3; i++) dist += (q[i] - p[i]) * (q[i] - p[i]);
+            if (pq.size() == k && pq.top().key > dist) {
+                pq.pop();
+                pq.push(dist, nd.pidx);
+            } else if (pq.size() < k) {
+                pq.push(dist, nd.pidx);
+            }
+            return;
+        }
+        int nd1 = (dim + 1) % 3;
+        double tmp;
+        if (q[dim] <= nd.key) {
+            tmp = bmax[dim]; bmax[dim] = nd.key;
+            search(q, k, nd.left, nd1, bmin, bmax, pq);
+            bmax[dim] = tmp;
+        } else {
+            tmp = bmin[dim]; bmin[dim] = nd.key;
+            search(q, k, nd.right, nd1, bmin, bmax, pq);
+            bmin[dim] = tmp;
+        }
+        if (q[dim] <= nd.key) {
+            tmp = bmin[dim]; bmin[dim] = nd.key;
+            if (overlap(q, k, bmin, bmax, pq)) search(q, k, nd.right, nd1, bmin, bmax, pq);
+            bmin[dim] = tmp;
+        } else {
+            tmp = bmax[dim]; bmax[dim] = nd.key;
+            if (overlap(q, k, bmin, bmax, pq)) search(q, k, nd.left, nd1, bmin, bmax, pq);
+            bmax[dim] = tmp;
+        }
+    }
+
+    const double* xyz_;
+    int n_;
+    std::vector<int> sorter_[3];
+    std::vector<char> side_;
+    std::vector<int> tmp_;
+    std::vector<TreeNode> nodes_;
+};
+
+// SplitMix64 (Steele, Lea, Flood 2014): a counter-based 64-bit mixer.
+inline uint64_t splitmix64(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+// uniform double in [-1, 1) from the top 53 bits
+inline double unit_sym(uint64_t r) { return (double)(r >> 11) * (2.0 / 9007199254740992.0) - 1.0; }
+
+}  // namespace
+
+extern "C" {
+
+int rbfx_knn_kdtree(int32_t n_nodes, const double* xyz, int32_t k, int32_t* out_idx, int32_t n_threads) {
+    if (n_nodes <= 0 || !xyz || !out_idx || k <= 0) return RBFX_E_INVALID;
+    if (k > n_nodes) return RBFX_E_INVALID;  // caller clamps like Grid::checkStencilSize
+    KdTree3 tree(xyz, n_nodes);
+#ifdef _OPENMP
+    int nt = n_threads > 0 ? n_threads : omp_get_max_threads();
+#pragma omp parallel num_threads(nt)
+#endif
+    {
+        MaxHeapD pq;
+#ifdef _OPENMP
+#pragma omp for schedule(static, 1024)
+#endif
+        for (int32_t i = 0; i < n_nodes; i++) tree.knn(xyz + 3 * (size_t)i, k, pq, out_idx + (size_t)i * k);
+    }
+    (void)n_threads;
+    return RBFX_OK;
+}
+
+int rbfx_nodes_fibonacci(int32_t n_nodes, double jitter, uint64_t seed, double* xyz) {
+    if (n_nodes <= 0 || !xyz) return RBFX_E_INVALID;
+    const double golden = M_PI * (1.0 + std::sqrt(5.0));
+    for (int32_t i = 0; i < n_nodes; i++) {
+        double t = (double)i + 0.5;
+        double z = 1.0 - 2.0 * t / (double)n_nodes;
+        double s = std::sqrt(std::max(0.0, 1.0 - z * z));
+        double ph = golden * t;
+        double p[3] = {s * std::cos(ph), s * std::sin(ph), z};
+        for (int d = 0; d < 3; d++) {
+            double j = 0.0;
+            if (jitter != 0.0) j = jitter * unit_sym(splitmix64(seed * 0x100000001B3ull + 3ull * (uint64_t)i + (uint64_t)d));
+            xyz[3 * (size_t)i + d] = p[d] + j;
+        }
+    }
+    return RBFX_OK;
+}
+
+}  // extern "C"

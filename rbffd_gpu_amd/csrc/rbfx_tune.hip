@@ -1,0 +1,86 @@
+// Launch-shape selection for the fused ELL kernels (fields per thread, block size).
+//
+// A tuning table maps (kind, rows bucket, width, n_ops, fields, storage, field type,
+// accumulation) to the shape measured fastest by rbfx_tune_spmm / the autotuner in the
+// Python layer; the table lives in memory and can be saved/loaded as text so a run can reuse
+// an earlier sweep. Without an entry the heuristic picks the largest fields-per-thread of
+// 8/4/2/1 that divides F and keeps at most 16 FP64 (32 FP32) accumulators per thread.
+#include <cstdio>
+#include <map>
+#include <mutex>
+#include <tuple>
+
+#include "rbfx_ell_common.h"
+
+namespace {
+using Key = std::tuple<int, int, int, int, int, int, int, int>;
+std::map<Key, EllTune> g_table;
+std::mutex g_mu;
+
+int rows_bucket(int n) {
+    int b = 0;
+    while ((1 << (b + 1)) <= n) b++;
+    return b;
+}
+
+Key make_key(const rbfx_ell* e, int F, int ft, int acc, int kind) {
+    return Key(kind, rows_bucket(e->n_rows), e->width, e->n_ops, F, e->storage, ft, acc);
+}
+
+bool valid(int F, int fpt, int block) {
+    if (fpt <= 0 || F % fpt) return false;
+    if (fpt != 1 && fpt != 2 && fpt != 4 && fpt != 8) return false;
+    if (block < 64 || block > 256 || block % 64) return false;
+    return (F / fpt) <= block;
+}
+}  // namespace
+
+int rbfx_ell_pick(rbfx_ell* e, int F, int ft, int acc, int kind, EllTune* t) {
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        auto it = g_table.find(make_key(e, F, ft, acc, kind));
+        if (it != g_table.end()) {
+            *t = it->second;
+            return RBFX_OK;
+        }
+    }
+    int max_acc = (acc == RBFX_F64) ? 16 : 32;
+    int fpt = 1;
+    for (int c : {8, 4, 2}) {
+        if (F % c == 0 && c * e->n_ops <= max_acc) { fpt = c; break; }
+    }
+    t->fpt = fpt;
+    t->block = 256;
+    if (!valid(F, t->fpt, t->block))
+        return rbfx_fail(e->ctx, RBFX_E_TOO_LARGE, "no launch shape for %d fields (at most 256*8)", F);
+    return RBFX_OK;
+}
+
+extern "C" {
+
+// Sets the launch shape used for (ell, n_fields, types, kind); kind 0 = SpMM, 1 = RK4 stage.
+int rbfx_tune_set(rbfx_ell* e, int32_t kind, int32_t n_fields, int32_t field_type, int32_t accum, int32_t fpt, int32_t block) {
+    if (!e) return RBFX_E_INVALID;
+    if (!valid(n_fields, fpt, block)) return rbfx_fail(e->ctx, RBFX_E_INVALID, "tune_set: invalid shape fpt %d block %d for %d fields", fpt, block, n_fields);
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_table[make_key(e, n_fields, field_type, accum, kind)] = EllTune{fpt, block};
+    return RBFX_OK;
+}
+
+int rbfx_tune_get(rbfx_ell* e, int32_t kind, int32_t n_fields, int32_t field_type, int32_t accum, int32_t* fpt, int32_t* block) {
+    if (!e || !fpt || !block) return RBFX_E_INVALID;
+    EllTune t;
+    int rc = rbfx_ell_pick(e, n_fields, field_type, accum, kind, &t);
+    if (rc) return rc;
+    *fpt = t.fpt;
+    *block = t.block;
+    return RBFX_OK;
+}
+
+int rbfx_tune_clear(void) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_table.clear();
+    return RBFX_OK;
+}
+
+}  // extern "C"

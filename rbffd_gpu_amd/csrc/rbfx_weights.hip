@@ -1,0 +1,788 @@
+// Batched RBF-FD weight computation: one small dense saddle-point solve per stencil.
+//
+// Reference: RBFFD::computeAllWeightsForStencil_Direct (src/rbffd/rbffd.cpp:225-352) builds
+//   M = [A P; P' 0], A(j,i) = phi_eps(x_i - x_j)   (distanceMatrix, rbffd.cpp:888-969)
+//   R(:,t) = L_t phi(x_centre - x_j), L_t p(x_centre) for the polynomial rows (356-537)
+// and calls arma::solve (LAPACK dgesv: LU with partial pivoting). HV columns are scaled by
+// getHVScalar (295-297). Gaussian phi and its derivatives follow rbf_gaussian.h:40-249.
+//
+// GPU design: one workgroup per stencil. The node coordinates of the stencil are staged in
+// LDS, the augmented matrix [M | R] is assembled straight into LDS, factored in place by
+// right-looking elimination with implicit partial pivoting (pivot = largest |a_rk| among the
+// rows not yet used; the row order is kept in a permutation instead of moving rows), and
+// back-substituted. Stencils are grouped by system size so each launch sizes LDS for its
+// class. FP32 mode factors a float copy and refines x with FP64 residuals recomputed from the
+// node coordinates; stencils whose refinement does not converge are re-solved in FP64.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "rbfx_internal.h"
+
+namespace {
+
+constexpr int WBLOCK = 256;
+constexpr int MAX_M = 128;
+constexpr int MAX_T = 16;
+constexpr size_t LDS_LIMIT = 160 * 1024;
+
+struct Mono3 {
+    int8_t a, b, c;
+};
+
+// Monomials of degree 0..3 in centred coordinates; order: by degree, then descending power of
+// x, then of y (the same list as the CPU restatement used by the tests).
+__constant__ Mono3 c_mono[20] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {2, 0, 0}, {1, 1, 0}, {1, 0, 1},
+                                 {0, 2, 0}, {0, 1, 1}, {0, 0, 2}, {3, 0, 0}, {2, 1, 0}, {2, 0, 1}, {1, 2, 0},
+                                 {1, 1, 1}, {1, 0, 2}, {0, 3, 0}, {0, 2, 1}, {0, 1, 2}, {0, 0, 3}};
+
+int num_monomials(int deg) { return deg < 0 ? 0 : (deg + 1) * (deg + 2) * (deg + 3) / 6; }
+
+struct WParams {
+    const int32_t* st_off;
+    const int32_t* st_idx;
+    const double* xyz;
+    const double* eps;
+    const int32_t* order;  // stencil ids of this size class
+    int count;
+    int m;                 // system size of the class
+    int np;
+    uint32_t types[MAX_T];
+    int T;
+    int dim, hv_k;
+    double hv_scale;
+    int nnz;
+    double* w;
+    double* diag;
+    int* fail_flags;       // per stencil: 1 = singular / not converged (FP32 mode)
+    int max_iter;
+    int fp32;
+};
+
+__device__ __forceinline__ double sq3(double x, double y, double z) { return x * x + y * y + z * z; }
+
+// Gaussian RBF (rbf_gaussian.h) at diff = (x,y,z); e2 = eps^2.
+__device__ double g_eval(double e2, double x, double y, double z) { return exp(-(sq3(x, y, z) * e2)); }
+
+__device__ double g_hv_pk(double e2, int dim, int k, double r2) {
+    // p_0 = 1, p_1 = 4 e2 r2 - 2d, p_k = 4(e2 r2 - 2(k-1) - d/2) p_{k-1} - 8(k-1)(2(k-1) - 2 + d) p_{k-2}
+    double e2r2 = e2 * r2;
+    if (k == 0) return 1.;
+    double pm2 = 1., pm1 = 4. * (e2r2)-2. * dim;
+    for (int q = 2; q <= k; q++) {
+        double pk = 4. * (e2r2 - 2. * (q - 1) - dim / 2.) * pm1 - 8. * (q - 1) * (2. * (q - 1) - 2. + dim) * pm2;
+        pm2 = pm1;
+        pm1 = pk;
+    }
+    return pm1;
+}
+
+__device__ void g_sph(double x, double y, double z, double& lam, double& th) {
+    lam = atan2(y, x);
+    th = atan2(z, sqrt(x * x + y * y));
+}
+
+// R entry of stencil node j for operator `type` (rbffd.cpp:356-537), diff = x0 - xj.
+__device__ double rbf_rhs(uint32_t type, double eps, double e2, int dim, int hv_k, const double* x0, const double* xj,
+                          double lam0, double th0, double lamj, double thj) {
+    double dx = x0[0] - xj[0], dy = x0[1] - xj[1], dz = x0[2] - xj[2];
+    double ev = g_eval(e2, dx, dy, dz);
+    switch (type) {
+        case RBFX_X: return -2. * (dx * e2) * ev;
+        case RBFX_Y: return -2. * (dy * e2) * ev;
+        case RBFX_Z: return -2. * (dz * e2) * ev;
+        case RBFX_LAPL:
+            if (dim == 1) return 2. * e2 * (-1. + 2. * (dx * dx * e2)) * ev;
+            if (dim == 2) return 4. * e2 * (-1. + dx * dx * e2 + dy * dy * e2) * ev;
+            return (-6. * e2 + 4. * (sq3(dx, dy, dz) * (e2 * e2))) * ev;
+        case RBFX_INTERP: return ev;
+        case RBFX_R: return -2. * (sqrt(sq3(dx, dy, dz)) * e2) * ev;
+        case RBFX_LAMBDA: {
+            double d = cos(thj) * sin(lam0 - lamj);
+            return (fabs(d) > 0.) ? d * (-2. * e2 * ev) : 0.;
+        }
+        case RBFX_THETA: {
+            double d = cos(thj) * sin(th0) * cos(lam0 - lamj) - sin(thj) * cos(th0);
+            return (fabs(d) > 0.) ? d * (-2. * e2 * ev) : 0.;
+        }
+        case RBFX_LSFC: {
+            double r2 = sq3(dx, dy, dz);
+            double dr = -2. * e2 * ev;
+            double d2 = dr + 4. * (e2 * e2) * r2 * exp(-(e2 * r2));
+            return (1. / 4.) * ((4. - r2) * d2 + (4. - 3. * r2) * dr);
+        }
+        case RBFX_XSFC: case RBFX_YSFC: case RBFX_ZSFC: {
+            double xtx = x0[0] * xj[0] + x0[1] * xj[1] + x0[2] * xj[2];
+            int c = (type == RBFX_XSFC) ? 0 : (type == RBFX_YSFC ? 1 : 2);
+            return (x0[c] * xtx - xj[c]) * (-2. * e2 * ev);
+        }
+        case RBFX_HV: return pow(eps, 2 * hv_k) * g_hv_pk(e2, dim, hv_k, sq3(dx, dy, dz)) * ev;
+        default: return 0.;
+    }
+}
+
+__device__ double mono_eval(int p, double dx, double dy, double dz) {
+    Mono3 mm = c_mono[p];
+    double v = 1.0;
+    for (int i = 0; i < mm.a; i++) v *= dx;
+    for (int i = 0; i < mm.b; i++) v *= dy;
+    for (int i = 0; i < mm.c; i++) v *= dz;
+    return v;
+}
+
+// Operator applied to monomial p at the centre (d = 0): only degree 0..2 terms contribute.
+__device__ double mono_rhs(uint32_t type, int p, const double* x0, int dim, int hv_k, double lam0, double th0) {
+    Mono3 mm = c_mono[p];
+    int deg = mm.a + mm.b + mm.c;
+    double p0 = (deg == 0) ? 1.0 : 0.0;
+    double g[3] = {0, 0, 0};
+    double H[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    if (deg == 1) {
+        g[0] = mm.a; g[1] = mm.b; g[2] = mm.c;
+    } else if (deg == 2) {
+        int e[3] = {mm.a, mm.b, mm.c};
+        for (int i = 0; i < 3; i++) {
+            if (e[i] == 2) H[i][i] = 2.0;
+            for (int j = i + 1; j < 3; j++)
+                if (e[i] == 1 && e[j] == 1) H[i][j] = H[j][i] = 1.0;
+        }
+    }
+    double tr = 0;
+    for (int i = 0; i < dim && i < 3; i++) tr += H[i][i];
+    switch (type) {
+        case RBFX_X: return g[0];
+        case RBFX_Y: return g[1];
+        case RBFX_Z: return g[2];
+        case RBFX_LAPL: return tr;
+        case RBFX_HV: return (hv_k == 0) ? p0 : (hv_k == 1 ? tr : 0.0);
+        case RBFX_LAMBDA: return -sin(lam0) * g[0] + cos(lam0) * g[1];
+        case RBFX_THETA: return -sin(th0) * cos(lam0) * g[0] - sin(th0) * sin(lam0) * g[1] + cos(th0) * g[2];
+        case RBFX_LSFC: {
+            double full = H[0][0] + H[1][1] + H[2][2];
+            double xhx = 0, xg = 0;
+            for (int i = 0; i < 3; i++) {
+                xg += x0[i] * g[i];
+                for (int j = 0; j < 3; j++) xhx += x0[i] * H[i][j] * x0[j];
+            }
+            return full - xhx - 2.0 * xg;
+        }
+        case RBFX_XSFC: case RBFX_YSFC: case RBFX_ZSFC: {
+            int c = (type == RBFX_XSFC) ? 0 : (type == RBFX_YSFC ? 1 : 2);
+            double xg = x0[0] * g[0] + x0[1] * g[1] + x0[2] * g[2];
+            return g[c] - x0[c] * xg;
+        }
+        case RBFX_INTERP: return p0;
+        default: return 0.0;
+    }
+}
+
+// ---- shared-memory layout of one system ----------------------------------------------------
+// [X: n*3 f64][SPH: n*2 f64][XS: m*T f64][BS: m*T f64][perm, pos: m i32 each][red: 8 f64 + 8 i32]
+// [Aug: m*ld of TA]; Aug = [M | R] with ld = m + T (+1 when even, so column walks spread over
+// the LDS banks). XS/BS (solution by unknown, original RHS by row) exist only in FP32 mode.
+__host__ __device__ inline int aug_ld(int m, int T) {
+    int ld = m + T;
+    return (ld & 1) ? ld : ld + 1;
+}
+
+__host__ __device__ inline size_t lds_head(int m, int n, int T, bool ir) {
+    size_t b = (size_t)n * 5 * sizeof(double) + (ir ? 2 * (size_t)m * T * sizeof(double) : 0) +
+               2 * (size_t)m * sizeof(int) + 8 * sizeof(double) + 8 * sizeof(int);
+    return (b + 15) / 16 * 16;
+}
+
+template <typename TA>
+__host__ __device__ inline size_t lds_bytes(int m, int n, int T, bool ir) {
+    return lds_head(m, n, T, ir) + (size_t)m * aug_ld(m, T) * sizeof(TA);
+}
+
+// Entry (r, c) of the FP64 saddle-point matrix, from the staged coordinates.
+__device__ __forceinline__ double m_entry(int r, int c, int n, double e2, const double* X) {
+    if (r < n && c < n)
+        return g_eval(e2, X[3 * c] - X[3 * r], X[3 * c + 1] - X[3 * r + 1], X[3 * c + 2] - X[3 * r + 2]);
+    if (r >= n && c >= n) return 0.0;
+    int i = (r < n) ? r : c;
+    int p = (r < n) ? c - n : r - n;
+    return mono_eval(p, X[3 * i] - X[0], X[3 * i + 1] - X[1], X[3 * i + 2] - X[2]);
+}
+
+struct Sys {
+    const WParams* P;
+    int s, off, n, m, T, ld;
+    double eps, e2;
+    bool sph;
+    double* X;
+    double* SPH;
+    double* XS;
+    double* BS;
+    int* perm;
+    int* pos;
+    double* red_v;
+    int* red_i;
+};
+
+__device__ __forceinline__ double rhs_entry(const Sys& S, int r, int t) {
+    const WParams& P = *S.P;
+    double lam0 = S.sph ? S.SPH[0] : 0., th0 = S.sph ? S.SPH[1] : 0.;
+    if (r < S.n)
+        return rbf_rhs(P.types[t], S.eps, S.e2, P.dim, P.hv_k, S.X, S.X + 3 * r, lam0, th0, S.sph ? S.SPH[2 * r] : 0.,
+                       S.sph ? S.SPH[2 * r + 1] : 0.);
+    return mono_rhs(P.types[t], r - S.n, S.X, P.dim, P.hv_k, lam0, th0);
+}
+
+// argmax over the block (ties -> smaller row); result in *out_p, visible after the caller's barrier.
+__device__ __forceinline__ void block_argmax(double v, int r, double* red_v, int* red_i, int* out_p) {
+    for (int off = 32; off > 0; off >>= 1) {
+        double ov = __shfl_xor(v, off, 64);
+        int oi = __shfl_xor(r, off, 64);
+        if (ov > v || (ov == v && oi < r)) { v = ov; r = oi; }
+    }
+    int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) { red_v[wv] = v; red_i[wv] = r; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double bv = red_v[0];
+        int bi = red_i[0];
+        for (int q = 1; q < (int)(blockDim.x >> 6); q++)
+            if (red_v[q] > bv || (red_v[q] == bv && red_i[q] < bi)) { bv = red_v[q]; bi = red_i[q]; }
+        *out_p = bi;
+    }
+}
+
+// Right-looking elimination of [M | R] with implicit partial pivoting. Row perm[k] is the
+// pivot row of step k (pos[] is the inverse); the multiplier of row r at step k is kept in
+// Aug[r][k] so later right-hand sides can be eliminated. Returns false on an exact zero pivot.
+template <typename TA>
+__device__ bool factor(Sys& S, TA* Aug, int* s_p) {
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int lane = tid & 63, wv = tid >> 6, nw = nthr >> 6;
+    const int m = S.m, ld = S.ld, wcols = m + S.T;
+    for (int r = tid; r < m; r += nthr) S.pos[r] = m;  // m = not yet pivoted
+    __syncthreads();
+    for (int k = 0; k < m; k++) {
+        double v = -1.0;
+        int r0 = 0x7fffffff;
+        for (int r = tid; r < m; r += nthr) {
+            if (S.pos[r] == m) {
+                double a = fabs((double)Aug[(size_t)r * ld + k]);
+                if (a > v || (a == v && r < r0)) { v = a; r0 = r; }
+            }
+        }
+        block_argmax(v, r0, S.red_v, S.red_i, s_p);
+        __syncthreads();
+        const int p = *s_p;
+        TA piv = Aug[(size_t)p * ld + k];
+        if (piv == (TA)0) return false;  // same LDS value for all threads: uniform exit
+        if (tid == 0) {
+            S.perm[k] = p;
+            S.pos[p] = k;
+        }
+        TA inv = (TA)1 / piv;
+        // wave wv eliminates the unpivoted rows wv, wv+nw, ...; row p is excluded by index,
+        // so the concurrent write of pos[p] above is never read in this step
+        for (int r = wv; r < m; r += nw) {
+            if (r == p || S.pos[r] != m) continue;
+            TA l = Aug[(size_t)r * ld + k] * inv;
+            for (int c = k + 1 + lane; c < wcols; c += 64) Aug[(size_t)r * ld + c] -= l * Aug[(size_t)p * ld + c];
+            if (lane == 0) Aug[(size_t)r * ld + k] = l;
+        }
+        __syncthreads();
+    }
+    return true;
+}
+
+// Back substitution on the eliminated RHS columns; x_k overwrites Aug[perm[k]][m + t].
+template <typename TA>
+__device__ void back_solve(Sys& S, TA* Aug) {
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int m = S.m, ld = S.ld, T = S.T;
+    for (int k = m - 1; k >= 0; k--) {
+        int p = S.perm[k];
+        if (tid < T) {
+            size_t e = (size_t)p * ld + m + tid;
+            Aug[e] = Aug[e] / Aug[(size_t)p * ld + k];
+        }
+        __syncthreads();
+        for (int q = tid; q < k * T; q += nthr) {
+            int kk = q / T, t = q - kk * T;
+            int r = S.perm[kk];
+            Aug[(size_t)r * ld + m + t] -= Aug[(size_t)r * ld + k] * Aug[(size_t)p * ld + m + t];
+        }
+        __syncthreads();
+    }
+}
+
+// Forward elimination of new RHS columns with the stored multipliers.
+template <typename TA>
+__device__ void forward_solve(Sys& S, TA* Aug) {
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int m = S.m, ld = S.ld, T = S.T;
+    for (int k = 0; k < m - 1; k++) {
+        int p = S.perm[k];
+        for (int q = tid; q < m * T; q += nthr) {
+            int r = q / T, t = q - r * T;
+            if (S.pos[r] > k) Aug[(size_t)r * ld + m + t] -= Aug[(size_t)r * ld + k] * Aug[(size_t)p * ld + m + t];
+        }
+        __syncthreads();
+    }
+}
+
+// r = b - M x in FP64 with M rebuilt from the coordinates; x(k,t) = XS[k*T+t] (by unknown).
+// Writes r into Aug's RHS columns (as TA) and returns, through red, max|r|, max|x|, max|b|
+// and ||M||_inf for the backward error. One thread per row.
+template <typename TA>
+__device__ void residual(Sys& S, TA* Aug, const double* XS, double* out_be) {
+    const int tid = threadIdx.x;
+    const int m = S.m, ld = S.ld, T = S.T;
+    double rmax[MAX_T], xmax[MAX_T], bmax[MAX_T];
+    for (int t = 0; t < T; t++) rmax[t] = xmax[t] = bmax[t] = 0.0;
+    double rowsum = 0.0;
+    if (tid < m) {
+        double acc[MAX_T];
+        for (int t = 0; t < T; t++) {
+            double b = S.BS ? S.BS[tid * T + t] : rhs_entry(S, tid, t);
+            acc[t] = b;
+            bmax[t] = fabs(b);
+        }
+        for (int c = 0; c < m; c++) {
+            double a = m_entry(tid, c, S.n, S.e2, S.X);
+            rowsum += fabs(a);
+            for (int t = 0; t < T; t++) acc[t] -= a * XS[c * T + t];
+        }
+        for (int t = 0; t < T; t++) {
+            rmax[t] = fabs(acc[t]);
+            xmax[t] = fabs(XS[tid * T + t]);
+            if (Aug) Aug[(size_t)tid * ld + m + t] = (TA)acc[t];
+        }
+    }
+    // block max-reductions of rowsum and the per-column maxima
+    __shared__ double s_red[4 * MAX_T + 1];
+    if (tid < 3 * T + 1) s_red[tid] = 0.0;
+    __syncthreads();
+    for (int off = 32; off > 0; off >>= 1) {
+        rowsum = fmax(rowsum, __shfl_xor(rowsum, off, 64));
+        for (int t = 0; t < T; t++) {
+            rmax[t] = fmax(rmax[t], __shfl_xor(rmax[t], off, 64));
+            xmax[t] = fmax(xmax[t], __shfl_xor(xmax[t], off, 64));
+            bmax[t] = fmax(bmax[t], __shfl_xor(bmax[t], off, 64));
+        }
+    }
+    if ((tid & 63) == 0) {
+        // values are non-negative: integer max on the bit pattern is the float max
+        atomicMax((unsigned long long*)&s_red[0], (unsigned long long)__double_as_longlong(rowsum));
+        for (int t = 0; t < T; t++) {
+            atomicMax((unsigned long long*)&s_red[1 + t], (unsigned long long)__double_as_longlong(rmax[t]));
+            atomicMax((unsigned long long*)&s_red[1 + T + t], (unsigned long long)__double_as_longlong(xmax[t]));
+            atomicMax((unsigned long long*)&s_red[1 + 2 * T + t], (unsigned long long)__double_as_longlong(bmax[t]));
+        }
+    }
+    __syncthreads();
+    double be = 0.0;
+    for (int t = 0; t < T; t++) be = fmax(be, s_red[1 + t] / (s_red[0] * s_red[1 + T + t] + s_red[1 + 2 * T + t]));
+    *out_be = be;
+    __syncthreads();
+}
+
+__device__ void stage_nodes(Sys& S) {
+    const WParams& P = *S.P;
+    for (int j = threadIdx.x; j < S.n; j += blockDim.x) {
+        const double* p = P.xyz + 3 * (size_t)P.st_idx[S.off + j];
+        S.X[3 * j] = p[0];
+        S.X[3 * j + 1] = p[1];
+        S.X[3 * j + 2] = p[2];
+        if (S.sph) {
+            double lam, th;
+            g_sph(p[0], p[1], p[2], lam, th);
+            S.SPH[2 * j] = lam;
+            S.SPH[2 * j + 1] = th;
+        }
+    }
+}
+
+template <typename TA>
+__device__ TA* setup(Sys& S, const WParams& P, unsigned char* smem, int s, bool ir) {
+    S.P = &P;
+    S.s = s;
+    S.off = P.st_off[s];
+    S.n = P.st_off[s + 1] - S.off;
+    S.m = P.m;
+    S.T = P.T;
+    S.ld = aug_ld(S.m, S.T);
+    S.eps = P.eps[s];
+    S.e2 = S.eps * S.eps;
+    S.sph = false;
+    for (int t = 0; t < S.T; t++) S.sph |= (P.types[t] == RBFX_LAMBDA || P.types[t] == RBFX_THETA);
+    double* d = (double*)smem;
+    S.X = d;
+    S.SPH = S.X + 3 * S.n;
+    double* q = S.SPH + 2 * S.n;
+    if (ir) {
+        S.XS = q;
+        S.BS = q + S.m * S.T;
+        q += 2 * S.m * S.T;
+    } else {
+        S.XS = S.BS = nullptr;
+    }
+    S.perm = (int*)q;
+    S.pos = S.perm + S.m;
+    S.red_v = (double*)(S.pos + S.m);
+    S.red_i = (int*)(S.red_v + 8);
+    return (TA*)(smem + lds_head(S.m, S.n, S.T, ir));
+}
+
+__device__ void write_weights(const Sys& S, const WParams& P, const double* XS) {
+    for (int q = threadIdx.x; q < S.n * S.T; q += blockDim.x) {
+        int j = q / S.T, t = q - j * S.T;
+        double scale = (P.types[t] == RBFX_HV) ? P.hv_scale : 1.0;
+        P.w[(size_t)t * P.nnz + S.off + j] = XS[j * S.T + t] * scale;
+    }
+}
+
+__device__ void write_failed(const Sys& S, const WParams& P) {
+    for (int q = threadIdx.x; q < S.n * S.T; q += blockDim.x) {
+        int j = q / S.T, t = q - j * S.T;
+        P.w[(size_t)t * P.nnz + S.off + j] = NAN;
+    }
+    if (threadIdx.x == 0) {
+        P.fail_flags[S.s] = 1;
+        if (P.diag) {
+            P.diag[4 * (size_t)S.s] = NAN;
+            P.diag[4 * (size_t)S.s + 1] = 0;
+            P.diag[4 * (size_t)S.s + 2] = 0;
+            P.diag[4 * (size_t)S.s + 3] = RBFX_PATH_SINGULAR;
+        }
+    }
+}
+
+// max|U| / max|M| over the factored rows (growth factor of the elimination).
+template <typename TA>
+__device__ double growth(const Sys& S, const TA* Aug) {
+    double umax = 0.0, amax = 0.0;
+    for (int q = threadIdx.x; q < S.m * S.m; q += blockDim.x) {
+        int r = q / S.m, c = q - r * S.m;
+        if (c >= S.pos[r]) umax = fmax(umax, fabs((double)Aug[(size_t)r * S.ld + c]));
+        amax = fmax(amax, fabs(m_entry(r, c, S.n, S.e2, S.X)));
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        umax = fmax(umax, __shfl_xor(umax, off, 64));
+        amax = fmax(amax, __shfl_xor(amax, off, 64));
+    }
+    __shared__ double g_red[2];
+    if (threadIdx.x < 2) g_red[threadIdx.x] = 0.0;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) {
+        atomicMax((unsigned long long*)&g_red[0], (unsigned long long)__double_as_longlong(umax));
+        atomicMax((unsigned long long*)&g_red[1], (unsigned long long)__double_as_longlong(amax));
+    }
+    __syncthreads();
+    double gr = g_red[0] / g_red[1];
+    __syncthreads();
+    return gr;
+}
+
+// FP64 path: assemble, factor, back-substitute (the arma::solve restatement on the GPU).
+__global__ void __launch_bounds__(WBLOCK) weights_f64_kernel(WParams P) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ int s_p;
+    if ((int)blockIdx.x >= P.count) return;
+    const int s = P.order[blockIdx.x];
+    Sys S;
+    double* Aug = setup<double>(S, P, smem, s, false);
+    const int tid = threadIdx.x, nthr = blockDim.x, m = S.m, T = S.T, ld = S.ld;
+    stage_nodes(S);
+    __syncthreads();
+    for (int q = tid; q < m * m; q += nthr) {
+        int r = q / m, c = q - r * m;
+        Aug[(size_t)r * ld + c] = m_entry(r, c, S.n, S.e2, S.X);
+    }
+    for (int q = tid; q < m * T; q += nthr) {
+        int r = q / T, t = q - r * T;
+        Aug[(size_t)r * ld + m + t] = rhs_entry(S, r, t);
+    }
+    __syncthreads();
+    if (!factor<double>(S, Aug, &s_p)) {
+        write_failed(S, P);
+        return;
+    }
+    back_solve<double>(S, Aug);
+    for (int q = tid; q < S.n * T; q += nthr) {
+        int j = q / T, t = q - j * T;
+        double scale = (P.types[t] == RBFX_HV) ? P.hv_scale : 1.0;
+        P.w[(size_t)t * P.nnz + S.off + j] = Aug[(size_t)S.perm[j] * ld + m + t] * scale;
+    }
+    if (tid == 0) P.fail_flags[s] = 0;
+    if (P.diag) {
+        double gr = growth<double>(S, Aug);
+        // the factors are no longer needed: gather x by unknown into the head of Aug
+        // (through registers, so no thread overwrites a value another still has to read)
+        double* xs = Aug;
+        double tmp[MAX_T];
+        if (tid < m)
+            for (int t = 0; t < T; t++) tmp[t] = Aug[(size_t)S.perm[tid] * ld + m + t];
+        __syncthreads();
+        if (tid < m)
+            for (int t = 0; t < T; t++) xs[tid * T + t] = tmp[t];
+        __syncthreads();
+        double be;
+        residual<double>(S, (double*)nullptr, xs, &be);
+        if (tid == 0) {
+            P.diag[4 * (size_t)s] = be;
+            P.diag[4 * (size_t)s + 1] = gr;
+            P.diag[4 * (size_t)s + 2] = 0;
+            P.diag[4 * (size_t)s + 3] = RBFX_PATH_F64;
+        }
+    }
+}
+
+// FP32 path: FP32 factorisation of [M | R], then x += LU^-1 (b - M x) with the residual in
+// FP64 (M and b rebuilt in FP64) until the correction stops shrinking or is below FP64
+// resolution. A stencil that does not converge is flagged (fail_flags = 2) for an FP64 re-solve.
+__global__ void __launch_bounds__(WBLOCK) weights_f32ir_kernel(WParams P) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ int s_p;
+    __shared__ double s_dn[2];
+    if ((int)blockIdx.x >= P.count) return;
+    const int s = P.order[blockIdx.x];
+    Sys S;
+    float* Aug = setup<float>(S, P, smem, s, true);
+    const int tid = threadIdx.x, nthr = blockDim.x, m = S.m, T = S.T, ld = S.ld;
+    stage_nodes(S);
+    __syncthreads();
+    for (int q = tid; q < m * m; q += nthr) {
+        int r = q / m, c = q - r * m;
+        Aug[(size_t)r * ld + c] = (float)m_entry(r, c, S.n, S.e2, S.X);
+    }
+    for (int q = tid; q < m * T; q += nthr) {
+        int r = q / T, t = q - r * T;
+        double b = rhs_entry(S, r, t);
+        S.BS[r * T + t] = b;
+        Aug[(size_t)r * ld + m + t] = (float)b;
+        S.XS[r * T + t] = 0.0;
+    }
+    __syncthreads();
+    if (!factor<float>(S, Aug, &s_p)) {
+        if (tid == 0) P.fail_flags[s] = 2;  // FP32 pivot underflow: let FP64 decide
+        return;
+    }
+    back_solve<float>(S, Aug);
+    double prev = INFINITY, be = INFINITY;
+    int it = 0;
+    bool ok = false;
+    for (;;) {
+        // x += d, where d(k,t) = Aug[perm[k]][m+t]
+        double dmax = 0.0, xmax = 0.0;
+        if (tid < m) {
+            int p = S.perm[tid];
+            for (int t = 0; t < T; t++) {
+                double d = (double)Aug[(size_t)p * ld + m + t];
+                double x = S.XS[tid * T + t] + d;
+                S.XS[tid * T + t] = x;
+                dmax = fmax(dmax, fabs(d));
+                xmax = fmax(xmax, fabs(x));
+            }
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            dmax = fmax(dmax, __shfl_xor(dmax, off, 64));
+            xmax = fmax(xmax, __shfl_xor(xmax, off, 64));
+        }
+        if (tid < 2) s_dn[tid] = 0.0;
+        __syncthreads();
+        if ((tid & 63) == 0) {
+            atomicMax((unsigned long long*)&s_dn[0], (unsigned long long)__double_as_longlong(dmax));
+            atomicMax((unsigned long long*)&s_dn[1], (unsigned long long)__double_as_longlong(xmax));
+        }
+        __syncthreads();
+        double rel = s_dn[0] / s_dn[1];
+        __syncthreads();
+        residual<float>(S, Aug, S.XS, &be);  // also loads r into the RHS columns
+        it++;
+        if (!(rel < 0.5 * prev) && it > 1) break;      // stagnation or divergence
+        if (rel <= 2.2e-16 || be <= 4.0 * 1.1e-16) { ok = true; break; }
+        if (it >= P.max_iter) break;
+        prev = rel;
+        forward_solve<float>(S, Aug);
+        back_solve<float>(S, Aug);
+    }
+    ok = ok || be <= 4.0 * 1.1e-16 * m;
+    if (!ok) {
+        if (tid == 0) P.fail_flags[s] = 2;
+        return;
+    }
+    write_weights(S, P, S.XS);
+    if (tid == 0) P.fail_flags[s] = 0;
+    if (P.diag) {
+        double gr = growth<float>(S, Aug);
+        if (tid == 0) {
+            P.diag[4 * (size_t)s] = be;
+            P.diag[4 * (size_t)s + 1] = gr;
+            P.diag[4 * (size_t)s + 2] = it;
+            P.diag[4 * (size_t)s + 3] = RBFX_PATH_F32_IR;
+        }
+    }
+}
+
+__global__ void mark_fallback_kernel(const int32_t* order, int count, double* diag) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count && diag) diag[4 * (size_t)order[i] + 3] = RBFX_PATH_F32_FALLBACK_F64;
+}
+
+}  // namespace
+
+extern "C" int rbfx_weights(rbfx_ctx* ctx, int32_t n_stencils, const int32_t* d_st_off, const int32_t* d_st_idx,
+                            int32_t n_nodes, const double* d_xyz, const double* d_eps, uint32_t dertypes, int32_t dim,
+                            int32_t poly_deg, int32_t hv_k, double hv_scale, int32_t mode, double* d_w, double* d_diag,
+                            int32_t* n_singular) {
+    if (!ctx) return RBFX_E_INVALID;
+    if (n_singular) *n_singular = 0;
+    if (n_stencils <= 0 || !d_st_off || !d_st_idx || n_nodes <= 0 || !d_xyz || !d_eps || !d_w)
+        return rbfx_fail(ctx, RBFX_E_INVALID, "weights: bad arguments");
+    const uint32_t supported = RBFX_X | RBFX_Y | RBFX_Z | RBFX_LAPL | RBFX_R | RBFX_HV | RBFX_LAMBDA | RBFX_THETA |
+                               RBFX_LSFC | RBFX_XSFC | RBFX_YSFC | RBFX_ZSFC | RBFX_INTERP;
+    if (!dertypes || (dertypes & ~supported))
+        return rbfx_fail(ctx, RBFX_E_INVALID, "weights: unsupported derivative mask 0x%x", dertypes);
+    if (dim < 1 || dim > 3) return rbfx_fail(ctx, RBFX_E_INVALID, "weights: dim %d not in 1..3", dim);
+    if (poly_deg < -1 || poly_deg > 3) return rbfx_fail(ctx, RBFX_E_INVALID, "weights: poly_deg %d not in -1..3", poly_deg);
+    if (hv_k < 0 || hv_k > 16) return rbfx_fail(ctx, RBFX_E_INVALID, "weights: hv_k %d", hv_k);
+    if (mode != RBFX_SOLVE_F64_LU && mode != RBFX_SOLVE_F32_LU_F64_IR) return rbfx_fail(ctx, RBFX_E_INVALID, "weights: mode %d", mode);
+    RBFX_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+
+    WParams P;
+    std::memset(&P, 0, sizeof(P));
+    int T = 0;
+    for (int b = 0; b < 16; b++)
+        if (dertypes & (1u << b)) P.types[T++] = 1u << b;
+    P.T = T;
+    P.np = num_monomials(poly_deg);
+    P.dim = dim;
+    P.hv_k = hv_k;
+    P.hv_scale = hv_scale;
+    P.st_off = d_st_off;
+    P.st_idx = d_st_idx;
+    P.xyz = d_xyz;
+    P.eps = d_eps;
+    P.w = d_w;
+    P.diag = d_diag;
+    P.max_iter = 8;
+
+    // sizes -> classes (host side), validating the CSR offsets and stencil indices
+    std::vector<int32_t> off(n_stencils + 1);
+    RBFX_HIP_CHECK(ctx, hipMemcpyAsync(off.data(), d_st_off, sizeof(int32_t) * (n_stencils + 1), hipMemcpyDeviceToHost, st));
+    RBFX_HIP_CHECK(ctx, hipStreamSynchronize(st));
+    if (off[0] != 0) return rbfx_fail(ctx, RBFX_E_INVALID, "weights: st_off[0] = %d, expected 0", off[0]);
+    int max_n = 0;
+    for (int s = 0; s < n_stencils; s++) {
+        int n = off[s + 1] - off[s];
+        if (n < 1) return rbfx_fail(ctx, RBFX_E_INVALID, "weights: stencil %d has size %d", s, n);
+        max_n = std::max(max_n, n);
+    }
+    if (max_n + P.np > MAX_M) return rbfx_fail(ctx, RBFX_E_TOO_LARGE, "weights: system size %d exceeds %d", max_n + P.np, MAX_M);
+    P.nnz = off[n_stencils];
+    {
+        std::vector<int32_t> idx(P.nnz);
+        RBFX_HIP_CHECK(ctx, hipMemcpyAsync(idx.data(), d_st_idx, sizeof(int32_t) * (size_t)P.nnz, hipMemcpyDeviceToHost, st));
+        RBFX_HIP_CHECK(ctx, hipStreamSynchronize(st));
+        for (int q = 0; q < P.nnz; q++)
+            if (idx[q] < 0 || idx[q] >= n_nodes) return rbfx_fail(ctx, RBFX_E_INVALID, "weights: stencil index %d out of range", idx[q]);
+    }
+    // counting sort of stencil ids by size, largest systems first
+    std::vector<int> cnt(MAX_M + 1, 0);
+    for (int s = 0; s < n_stencils; s++) cnt[off[s + 1] - off[s]]++;
+    std::vector<int> start(MAX_M + 2, 0);
+    for (int n = MAX_M; n >= 1; n--) start[n - 1] = start[n] + cnt[n];  // start[n-1]: first slot of size n
+    std::vector<int32_t> order(n_stencils);
+    {
+        std::vector<int> fill(MAX_M + 1);
+        for (int n = 1; n <= MAX_M; n++) fill[n] = start[n - 1] - cnt[n];
+        for (int s = 0; s < n_stencils; s++) {
+            int n = off[s + 1] - off[s];
+            order[fill[n]++] = s;
+        }
+    }
+    int32_t* d_order = nullptr;
+    int* d_flags = nullptr;
+    hipError_t e;
+    if ((e = hipMalloc(&d_order, sizeof(int32_t) * n_stencils)) != hipSuccess || (e = hipMalloc(&d_flags, sizeof(int) * n_stencils)) != hipSuccess) {
+        hipFree(d_order);
+        return rbfx_fail(ctx, RBFX_E_NOMEM, "weights: %s", hipGetErrorString(e));
+    }
+    int rc = RBFX_OK;
+    auto cleanup = [&]() {
+        hipStreamSynchronize(st);
+        hipFree(d_order);
+        hipFree(d_flags);
+    };
+    if ((e = hipMemcpyAsync(d_order, order.data(), sizeof(int32_t) * n_stencils, hipMemcpyHostToDevice, st)) != hipSuccess) {
+        cleanup();
+        return rbfx_fail(ctx, RBFX_E_HIP, "weights: %s", hipGetErrorString(e));
+    }
+    P.fail_flags = d_flags;
+
+    auto launch_classes = [&](bool ir, const int32_t* ord, const std::vector<int>& sizes_count,
+                              const std::vector<int>& sizes_start) -> int {
+        for (int n = MAX_M; n >= 1; n--) {
+            int c = sizes_count[n];
+            if (!c) continue;
+            WParams Q = P;
+            Q.order = ord + sizes_start[n];
+            Q.count = c;
+            Q.m = n + P.np;
+            Q.fp32 = ir ? 1 : 0;
+            size_t lds = ir ? lds_bytes<float>(Q.m, n, T, true) : lds_bytes<double>(Q.m, n, T, false);
+            if (lds > LDS_LIMIT) return rbfx_fail(ctx, RBFX_E_TOO_LARGE, "weights: system %d x %d with %d RHS needs %zu B of LDS", Q.m, Q.m, T, lds);
+            if (ir)
+                hipLaunchKernelGGL(weights_f32ir_kernel, dim3(c), dim3(WBLOCK), lds, st, Q);
+            else
+                hipLaunchKernelGGL(weights_f64_kernel, dim3(c), dim3(WBLOCK), lds, st, Q);
+            hipError_t le = hipGetLastError();
+            if (le != hipSuccess) return rbfx_fail(ctx, RBFX_E_HIP, "weights: launch (m=%d) failed: %s", Q.m, hipGetErrorString(le));
+        }
+        return RBFX_OK;
+    };
+    std::vector<int> cstart(MAX_M + 1, 0);
+    for (int n = 1; n <= MAX_M; n++) cstart[n] = start[n - 1] - cnt[n];
+    rc = launch_classes(mode == RBFX_SOLVE_F32_LU_F64_IR, d_order, cnt, cstart);
+    if (rc) { cleanup(); return rc; }
+
+    std::vector<int> flags(n_stencils);
+    if ((e = hipMemcpyAsync(flags.data(), d_flags, sizeof(int) * n_stencils, hipMemcpyDeviceToHost, st)) != hipSuccess ||
+        (e = hipStreamSynchronize(st)) != hipSuccess) {
+        cleanup();
+        return rbfx_fail(ctx, RBFX_E_HIP, "weights: %s", hipGetErrorString(e));
+    }
+    if (mode == RBFX_SOLVE_F32_LU_F64_IR) {
+        // FP64 re-solve of the stencils whose refinement did not converge
+        std::vector<int32_t> redo;
+        std::vector<int> rcnt(MAX_M + 1, 0), rstart(MAX_M + 1, 0);
+        for (int n = MAX_M; n >= 1; n--)
+            for (int q = cstart[n]; q < cstart[n] + cnt[n]; q++)
+                if (flags[order[q]] == 2) { redo.push_back(order[q]); rcnt[n]++; }
+        if (!redo.empty()) {
+            int acc = 0;
+            for (int n = MAX_M; n >= 1; n--) { rstart[n] = acc; acc += rcnt[n]; }
+            if ((e = hipMemcpyAsync(d_order, redo.data(), sizeof(int32_t) * redo.size(), hipMemcpyHostToDevice, st)) != hipSuccess) {
+                cleanup();
+                return rbfx_fail(ctx, RBFX_E_HIP, "weights: %s", hipGetErrorString(e));
+            }
+            rc = launch_classes(false, d_order, rcnt, rstart);
+            if (rc) { cleanup(); return rc; }
+            if (d_diag) {
+                hipLaunchKernelGGL(mark_fallback_kernel, dim3((unsigned)((redo.size() + 255) / 256)), dim3(256), 0, st, d_order, (int)redo.size(), d_diag);
+            }
+            std::vector<int> f2(n_stencils);
+            if ((e = hipMemcpyAsync(f2.data(), d_flags, sizeof(int) * n_stencils, hipMemcpyDeviceToHost, st)) != hipSuccess ||
+                (e = hipStreamSynchronize(st)) != hipSuccess) {
+                cleanup();
+                return rbfx_fail(ctx, RBFX_E_HIP, "weights: %s", hipGetErrorString(e));
+            }
+            for (int32_t s : redo) flags[s] = f2[s];
+        }
+    }
+    int sing = 0;
+    for (int s = 0; s < n_stencils; s++) sing += (flags[s] == 1);
+    cleanup();
+    if (n_singular) *n_singular = sing;
+    if (sing) return rbfx_fail(ctx, RBFX_E_SINGULAR, "weights: %d stencil system(s) singular", sing);
+    return RBFX_OK;
+}

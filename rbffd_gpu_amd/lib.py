@@ -1,0 +1,326 @@
+"""ctypes binding of librbfx.so (the C ABI in include/rbfx.h).
+
+Thin wrappers only: every computation runs in the library (HIP kernels on the GPU, or the
+host kd-tree / node generator). Errors from the library raise RbfxError with its message.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "librbfx.so")
+
+RBFX_OK = 0
+X, Y, Z, LAPL, R, HV, LAMBDA, THETA = 0x1, 0x2, 0x4, 0x8, 0x10, 0x20, 0x40, 0x80
+LSFC, XSFC, YSFC, ZSFC, INTERP = 0x100, 0x200, 0x400, 0x800, 0x8000
+F64, F32 = 0, 1
+SOLVE_F64_LU, SOLVE_F32_LU_F64_IR = 0, 1
+PATH_F64, PATH_F32_IR, PATH_F32_FALLBACK_F64, PATH_SINGULAR = 0, 1, 2, 3
+ERR_NAMES = {-1: "INVALID", -2: "HIP", -3: "NOMEM", -4: "NODEVICE", -5: "SINGULAR", -6: "STATE", -7: "TOO_LARGE"}
+
+
+class RbfxError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"rbfx error {code} ({ERR_NAMES.get(code, '?')}): {msg}")
+        self.code = code
+
+
+class _RK4Desc(C.Structure):
+    _fields_ = [
+        ("ell", C.c_void_p),
+        ("n_fields", C.c_int32),
+        ("field_storage", C.c_int32),
+        ("accum", C.c_int32),
+        ("d_coef", C.c_void_p * 16),
+        ("use_graph", C.c_int32),
+    ]
+
+
+_lib = None
+
+
+def load_library(path=None):
+    """Loads librbfx.so once; raises OSError if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise OSError(f"librbfx.so not found at {p}: run __graft_entry__.build() (make -C rbffd_gpu_amd)")
+    lib = C.CDLL(p)
+    vp, i32, u32, u64, f64 = C.c_void_p, C.c_int32, C.c_uint32, C.c_uint64, C.c_double
+    sig = {
+        "rbfx_create": (C.c_int, [C.c_int, C.POINTER(vp)]),
+        "rbfx_destroy": (None, [vp]),
+        "rbfx_last_error": (C.c_char_p, [vp]),
+        "rbfx_sync": (C.c_int, [vp]),
+        "rbfx_device_info": (C.c_int, [vp, C.c_char_p, i32, C.POINTER(i32), C.POINTER(i32)]),
+        "rbfx_dev_alloc": (C.c_int, [vp, u64, C.POINTER(vp)]),
+        "rbfx_dev_free": (C.c_int, [vp, vp]),
+        "rbfx_h2d": (C.c_int, [vp, vp, vp, u64]),
+        "rbfx_d2h": (C.c_int, [vp, vp, vp, u64]),
+        "rbfx_d2d": (C.c_int, [vp, vp, vp, u64]),
+        "rbfx_timer_start": (C.c_int, [vp]),
+        "rbfx_timer_stop": (C.c_int, [vp, C.POINTER(f64)]),
+        "rbfx_knn_kdtree": (C.c_int, [i32, vp, i32, vp, i32]),
+        "rbfx_nodes_fibonacci": (C.c_int, [i32, f64, u64, vp]),
+        "rbfx_weights": (C.c_int, [vp, i32, vp, vp, i32, vp, vp, u32, i32, i32, i32, f64, i32, vp, vp, C.POINTER(i32)]),
+        "rbfx_ell_create": (C.c_int, [vp, i32, i32, i32, vp, i32, vp, vp, i32, C.POINTER(vp)]),
+        "rbfx_ell_destroy": (None, [vp]),
+        "rbfx_ell_info": (C.c_int, [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)]),
+        "rbfx_spmv": (C.c_int, [vp, i32, vp, vp, i32, i32]),
+        "rbfx_spmm": (C.c_int, [vp, i32, vp, vp, i32, i32]),
+        "rbfx_rk4_create": (C.c_int, [vp, C.POINTER(_RK4Desc), C.POINTER(vp)]),
+        "rbfx_rk4_destroy": (None, [vp]),
+        "rbfx_rk4_set_state": (C.c_int, [vp, vp]),
+        "rbfx_rk4_get_state": (C.c_int, [vp, vp]),
+        "rbfx_rk4_step": (C.c_int, [vp, f64, i32]),
+        "rbfx_rk4_stage": (C.c_int, [vp, i32, f64]),
+        "rbfx_tune_set": (C.c_int, [vp, i32, i32, i32, i32, i32, i32]),
+        "rbfx_tune_get": (C.c_int, [vp, i32, i32, i32, i32, C.POINTER(i32), C.POINTER(i32)]),
+        "rbfx_tune_clear": (C.c_int, []),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+class _Api:
+    def __getattr__(self, name):
+        return getattr(load_library(), name)
+
+
+RBFX = _Api()
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def dertype_bits(mask):
+    """DerType bits of mask in ascending order (the weight column order)."""
+    return [1 << b for b in range(16) if mask & (1 << b)]
+
+
+def knn_kdtree(xyz, k, n_threads=0):
+    """kd-tree stencils (host C++ in librbfx): int32 array (N, k), row i starts with i."""
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    n = xyz.shape[0]
+    out = np.empty((n, k), dtype=np.int32)
+    rc = RBFX.rbfx_knn_kdtree(n, _ptr(xyz), k, _ptr(out), n_threads)
+    if rc:
+        raise RbfxError(rc, "knn_kdtree failed")
+    return out
+
+
+def nodes_fibonacci(n, jitter=1e-3, seed=0):
+    out = np.empty((n, 3), dtype=np.float64)
+    rc = RBFX.rbfx_nodes_fibonacci(n, jitter, seed, _ptr(out))
+    if rc:
+        raise RbfxError(rc, "nodes_fibonacci failed")
+    return out
+
+
+def csr_from_fixed(st):
+    st = np.ascontiguousarray(st, dtype=np.int32)
+    n, k = st.shape
+    return np.arange(0, (n + 1) * k, k, dtype=np.int32), st.reshape(-1)
+
+
+class Context:
+    def __init__(self, device=0):
+        lib = load_library()
+        h = C.c_void_p()
+        rc = lib.rbfx_create(device, C.byref(h))
+        if rc:
+            raise RbfxError(rc, lib.rbfx_last_error(None).decode())
+        self.h = h
+        self._buffers = []
+
+    def check(self, rc, what=""):
+        if rc:
+            raise RbfxError(rc, (what + ": " if what else "") + load_library().rbfx_last_error(self.h).decode())
+
+    def info(self):
+        name = C.create_string_buffer(256)
+        ncu, clk = C.c_int32(), C.c_int32()
+        self.check(RBFX.rbfx_device_info(self.h, name, 256, C.byref(ncu), C.byref(clk)))
+        return {"name": name.value.decode(), "n_cu": ncu.value, "clock_khz": clk.value}
+
+    def sync(self):
+        self.check(RBFX.rbfx_sync(self.h), "sync")
+
+    def alloc(self, shape, dtype):
+        return DeviceArray(self, shape, dtype)
+
+    def to_device(self, a):
+        a = np.ascontiguousarray(a)
+        d = DeviceArray(self, a.shape, a.dtype)
+        d.upload(a)
+        return d
+
+    def timer_start(self):
+        self.check(RBFX.rbfx_timer_start(self.h))
+
+    def timer_stop(self):
+        ms = C.c_double()
+        self.check(RBFX.rbfx_timer_stop(self.h, C.byref(ms)))
+        return ms.value
+
+    def weights(self, st_off, st_idx, xyz, eps, dertypes, dim=3, poly_deg=0, hv_k=0, hv_scale=1.0,
+                mode=SOLVE_F64_LU, diag=False, device_out=False):
+        """Batched weights. Host or device inputs; returns (w[T, nnz], diag[n_st, 4] or None)."""
+        d_off = st_off if isinstance(st_off, DeviceArray) else self.to_device(np.asarray(st_off, np.int32))
+        d_idx = st_idx if isinstance(st_idx, DeviceArray) else self.to_device(np.asarray(st_idx, np.int32))
+        d_xyz = xyz if isinstance(xyz, DeviceArray) else self.to_device(np.asarray(xyz, np.float64))
+        n_st = d_off.shape[0] - 1
+        e = np.asarray(eps, np.float64)
+        if e.ndim == 0:
+            e = np.full(n_st, float(e))
+        d_eps = self.to_device(e)
+        T = len(dertype_bits(dertypes))
+        nnz = d_idx.size
+        d_w = self.alloc((T, nnz), np.float64)
+        d_diag = self.alloc((n_st, 4), np.float64) if diag else None
+        nsing = C.c_int32()
+        rc = RBFX.rbfx_weights(self.h, n_st, d_off.ptr, d_idx.ptr, d_xyz.shape[0], d_xyz.ptr, d_eps.ptr, dertypes, dim,
+                               poly_deg, hv_k, hv_scale, mode, d_w.ptr, d_diag.ptr if d_diag else None, C.byref(nsing))
+        self.check(rc, "weights")
+        if device_out:
+            return d_w, d_diag
+        return d_w.download(), (d_diag.download() if d_diag else None)
+
+    def close(self):
+        for b in self._buffers:
+            b.free()
+        self._buffers = []
+        if self.h:
+            RBFX.rbfx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+class DeviceArray:
+    def __init__(self, ctx, shape, dtype):
+        self.ctx = ctx
+        self.shape = tuple(int(s) for s in (shape if isinstance(shape, (tuple, list)) else (shape,)))
+        self.dtype = np.dtype(dtype)
+        self.size = int(np.prod(self.shape)) if self.shape else 1
+        self.nbytes = self.size * self.dtype.itemsize
+        p = C.c_void_p()
+        ctx.check(RBFX.rbfx_dev_alloc(ctx.h, self.nbytes, C.byref(p)), "alloc")
+        self._p = p
+        ctx._buffers.append(self)
+
+    @property
+    def ptr(self):
+        return self._p
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.nbytes == self.nbytes, (a.nbytes, self.nbytes)
+        self.ctx.check(RBFX.rbfx_h2d(self.ctx.h, self._p, _ptr(a), self.nbytes), "h2d")
+
+    def download(self):
+        out = np.empty(self.shape, dtype=self.dtype)
+        self.ctx.check(RBFX.rbfx_d2h(self.ctx.h, _ptr(out), self._p, self.nbytes), "d2h")
+        return out
+
+    def free(self):
+        if self._p:
+            RBFX.rbfx_dev_free(self.ctx.h, self._p)
+            self._p = C.c_void_p()
+
+
+class EllOperators:
+    """Fused ELL operator set: n_ops operators sharing one (n_rows, width) index array."""
+
+    def __init__(self, ctx, idx, weights, n_cols=None, storage=F64, types=None):
+        self.ctx = ctx
+        d_idx = idx if isinstance(idx, DeviceArray) else ctx.to_device(np.asarray(idx, np.int32))
+        n_rows, width = d_idx.shape
+        d_ws = [w if isinstance(w, DeviceArray) else ctx.to_device(np.asarray(w, np.float64).reshape(n_rows, width))
+                for w in weights]
+        n_ops = len(d_ws)
+        ptrs = (C.c_void_p * n_ops)(*[w.ptr.value for w in d_ws])
+        tarr = np.array(types if types is not None else [0] * n_ops, dtype=np.uint32)
+        h = C.c_void_p()
+        rc = RBFX.rbfx_ell_create(ctx.h, n_rows, n_cols or n_rows, width, d_idx.ptr, n_ops, _ptr(tarr), ptrs, storage,
+                                  C.byref(h))
+        ctx.check(rc, "ell_create")
+        self.h = h
+        self.n_rows, self.width, self.n_ops, self.storage = n_rows, width, n_ops, storage
+
+    def spmv(self, op, d_u, d_out, field_type=F64, accum=F64):
+        self.ctx.check(RBFX.rbfx_spmv(self.h, op, d_u.ptr, d_out.ptr, field_type, accum), "spmv")
+
+    def spmm(self, n_fields, d_u, d_out, field_type=F64, accum=F64):
+        self.ctx.check(RBFX.rbfx_spmm(self.h, n_fields, d_u.ptr, d_out.ptr, field_type, accum), "spmm")
+
+    def tune_set(self, kind, n_fields, field_type, accum, fpt, block):
+        self.ctx.check(RBFX.rbfx_tune_set(self.h, kind, n_fields, field_type, accum, fpt, block), "tune_set")
+
+    def tune_get(self, kind, n_fields, field_type, accum):
+        f, b = C.c_int32(), C.c_int32()
+        self.ctx.check(RBFX.rbfx_tune_get(self.h, kind, n_fields, field_type, accum, C.byref(f), C.byref(b)), "tune_get")
+        return f.value, b.value
+
+    def close(self):
+        if self.h:
+            RBFX.rbfx_ell_destroy(self.h)
+            self.h = None
+
+
+class RK4:
+    def __init__(self, ctx, ell, n_fields, field_storage=F64, accum=F64, coefs=None, use_graph=True):
+        self.ctx = ctx
+        d = _RK4Desc()
+        d.ell = ell.h
+        d.n_fields = n_fields
+        d.field_storage = field_storage
+        d.accum = accum
+        self._coefs = []
+        for o in range(ell.n_ops):
+            c = None if coefs is None else coefs[o]
+            if c is not None:
+                dc = c if isinstance(c, DeviceArray) else ctx.to_device(np.asarray(c, np.float64))
+                self._coefs.append(dc)
+                d.d_coef[o] = dc.ptr.value
+        d.use_graph = 1 if use_graph else 0
+        h = C.c_void_p()
+        ctx.check(RBFX.rbfx_rk4_create(ctx.h, C.byref(d), C.byref(h)), "rk4_create")
+        self.h = h
+        self.n_rows = ell.n_rows
+        self.n_fields = n_fields
+
+    def set_state(self, u):
+        d = u if isinstance(u, DeviceArray) else self.ctx.to_device(np.asarray(u, np.float64))
+        self.ctx.check(RBFX.rbfx_rk4_set_state(self.h, d.ptr), "rk4_set_state")
+        self._state_in = d
+
+    def get_state(self):
+        d = self.ctx.alloc((self.n_rows, self.n_fields), np.float64)
+        self.ctx.check(RBFX.rbfx_rk4_get_state(self.h, d.ptr), "rk4_get_state")
+        out = d.download()
+        d.free()
+        return out
+
+    def step(self, dt, n_steps=1):
+        self.ctx.check(RBFX.rbfx_rk4_step(self.h, dt, n_steps), "rk4_step")
+
+    def stage(self, stage, dt):
+        self.ctx.check(RBFX.rbfx_rk4_stage(self.h, stage, dt), "rk4_stage")
+
+    def close(self):
+        if self.h:
+            RBFX.rbfx_rk4_destroy(self.h)
+            self.h = None
