@@ -1,0 +1,296 @@
+"""Benchmark: RK4 steps/s and SpMV HBM GB/s at N=1M, n=50 on one MI355X (FP32-stored vs FP64).
+
+Workload (the fused multi-operator multi-field RK4 configuration):
+  N = 1,000,000 nodes on the unit sphere (golden-angle spiral, +-1e-3 jitter, seed 0),
+  n = 50 kd-tree stencils, Gaussian RBF eps = 0.044 sqrt(N) - 0.14, constant monomial,
+  4 operators d/dx, d/dy, d/dz and hyperviscosity Delta^4 (dim 2, scale -2^-6 N^-4),
+  16 fields = seeded combinations of real spherical harmonics of degree <= 20,
+  PDE: solid-body rotation advection dq/dt = -(w x r).grad q + H q (one revolution per
+  unit time), classic RK4, dt = 1/4000.
+One "step" = one RK4 step of all 16 fields (4 fused stage kernels, replayed from a hipGraph)
+with every input resident in HBM. Stencils and weights are built on the device before the
+timed region (weights: batched GPU solve).
+
+Printed JSON: value = FP32-stored RK4 steps/s; extras carry the FP64 path, the SpMV GB/s,
+the FP32-vs-FP64 deviation and the error against the analytic rotated field.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def real_sph_harm(l, m, xyz):
+    """Real spherical harmonic Y_lm at the directions of xyz (orthonormal on the sphere)."""
+    from scipy.special import sph_harm_y
+
+    r = np.linalg.norm(xyz, axis=1)
+    theta = np.arccos(np.clip(xyz[:, 2] / r, -1, 1))
+    phi = np.arctan2(xyz[:, 1], xyz[:, 0])
+    if m == 0:
+        return sph_harm_y(l, 0, theta, phi).real
+    y = sph_harm_y(l, abs(m), theta, phi)
+    return np.sqrt(2) * (-1) ** m * (y.real if m > 0 else y.imag)
+
+
+def make_fields(xyz, F, seed=0, lmax=20, terms=6):
+    rng = np.random.default_rng(seed)
+    spec = []
+    u = np.zeros((xyz.shape[0], F))
+    for f in range(F):
+        ls = rng.integers(0, lmax + 1, size=terms)
+        ms = [int(rng.integers(-l, l + 1)) for l in ls]
+        cs = rng.standard_normal(terms)
+        spec.append(list(zip(ls.tolist(), ms, cs.tolist())))
+        for l, m, c in spec[-1]:
+            u[:, f] += c * real_sph_harm(int(l), int(m), xyz)
+    return u, spec
+
+
+def rotate(xyz, axis, angle):
+    k = axis / np.linalg.norm(axis)
+    c, s = np.cos(angle), np.sin(angle)
+    return xyz * c + np.cross(k, xyz) * s + np.outer(xyz @ k, k) * (1 - c)
+
+
+def bytes_per_stage(N, n, ops, F, s_w, s_f, n_coef, stage):
+    """Compulsory HBM bytes of one fused RK4 stage kernel launch:
+    weights + indices once, the gathered stage field once per node, the FP64 state u,
+    the FP64 accumulator (written in stage 1, read+written in 2-3, read in 4), the next
+    stage field (stage 4 of the FP64 path writes u only), the coefficient arrays."""
+    w = ops * N * n * s_w + N * n * 4
+    coef = n_coef * N * 8
+    f64 = s_f == 8
+    if stage == 1:
+        fld = s_f + 8 + 8 + s_f  # gather X (for FP64 that is u itself), u, acc write, X' write
+        if f64:
+            fld = 8 + 8 + 8  # u read once serves gather and state
+    elif stage in (2, 3):
+        fld = s_f + 8 + 16 + s_f
+    else:
+        fld = s_f + 8 + 8 + 8 + (0 if f64 else s_f)
+    return w + coef + F * N * fld
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--n-nodes", type=int, default=1000000)
+    ap.add_argument("--width", type=int, default=50)
+    ap.add_argument("--fields", type=int, default=16)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--kernel-only", action="store_true", help="set up, then launch only stage kernels (profiling)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is designed "
+                                   "(DESIGN.md) but not built", "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import rbffd_gpu_amd as R
+
+    N, n, F = args.n_nodes, args.width, args.fields
+    t0 = time.time()
+    xyz = R.nodes_fibonacci(N, 1e-3, 0)
+    st = R.knn_kdtree(xyz, n)
+    log(f"nodes + kd-tree stencils: {time.time() - t0:.1f}s")
+    ctx = R.Context(0)
+    info = ctx.info()
+    log("device", info)
+
+    # ---- weights on the GPU ----------------------------------------------------------------
+    mask = R.X | R.Y | R.Z | R.HV
+    eps = 0.044 * np.sqrt(N) - 0.14
+    hv_k, hv_gamma = 4, 2.0 ** -6
+    hv_scale = -hv_gamma * float(N) ** -hv_k
+    d_xyz = ctx.to_device(xyz)
+    off, idx = R.csr_from_fixed(st)
+    d_off, d_idx = ctx.to_device(off), ctx.to_device(idx)
+    ctx.sync()
+    t0 = time.time()
+    d_w, _ = ctx.weights(d_off, d_idx, d_xyz, eps, mask, dim=2, poly_deg=0, hv_k=hv_k, hv_scale=hv_scale,
+                         device_out=True)
+    ctx.sync()
+    t_weights = time.time() - t0
+    log(f"GPU weights ({N} stencils, n={n}, 4 RHS): {t_weights:.3f}s")
+    nnz = N * n
+    d_st = d_idx.view(0, (N, n))
+    w_views = [d_w.view(t * nnz, (N, n)) for t in range(4)]
+
+    # ---- operators, fields, coefficients ------------------------------------------------------
+    ell64 = R.EllOperators(ctx, d_st, w_views, storage=R.F64, types=R.dertype_bits(mask))
+    ell32 = R.EllOperators(ctx, d_st, w_views, storage=R.F32, types=R.dertype_bits(mask))
+    ell_sp64 = R.EllOperators(ctx, d_st, [w_views[0]], storage=R.F64)
+    ell_sp32 = R.EllOperators(ctx, d_st, [w_views[0]], storage=R.F32)
+    t0 = time.time()
+    u0, spec = make_fields(xyz, F, seed=0)
+    log(f"fields: {time.time() - t0:.1f}s")
+    axis = np.array([0.3, -0.2, 1.0])
+    omega = 2 * np.pi * axis / np.linalg.norm(axis)  # one revolution per unit time
+    vel = np.cross(omega, xyz)
+    coefs = [ctx.to_device(-vel[:, 0].copy()), ctx.to_device(-vel[:, 1].copy()), ctx.to_device(-vel[:, 2].copy()), None]
+    dt = 1.0 / 4000
+    d_u0 = ctx.to_device(u0)
+
+    def timed(rk, K, W):
+        rk.set_state(d_u0)
+        rk.step(dt, W)
+        ctx.sync()
+        t = time.perf_counter()
+        ctx.timer_start()
+        rk.step(dt, K)
+        ev_ms = ctx.timer_stop()
+        wall = time.perf_counter() - t
+        return wall, ev_ms
+
+    rk32 = R.RK4(ctx, ell32, F, R.F32, R.F64, coefs=coefs)
+    rk64 = R.RK4(ctx, ell64, F, R.F64, R.F64, coefs=coefs)
+
+    if args.kernel_only:
+        rk32.set_state(d_u0)
+        rk64.set_state(d_u0)
+        for _ in range(20):
+            for s in (1, 2, 3, 4):
+                rk32.stage(s, dt)
+        for _ in range(20):
+            for s in (1, 2, 3, 4):
+                rk64.stage(s, dt)
+        d_x = ctx.to_device(u0[:, 0].copy())
+        d_y = ctx.alloc(N, np.float64)
+        for _ in range(20):
+            ell_sp64.spmv(0, d_x, d_y)
+        ctx.sync()
+        log("kernel-only launches done")
+        return
+
+    K, W = args.steps, args.warmup
+    wall32, ev32 = timed(rk32, K, W)
+    u32 = rk32.get_state()
+    wall64, ev64 = timed(rk64, K, W)
+    u64 = rk64.get_state()
+    log(f"FP32-stored: {K / wall32:.1f} steps/s (wall) {K / (ev32 / 1e3):.1f} (events); "
+        f"FP64: {K / wall64:.1f} steps/s (wall) {K / (ev64 / 1e3):.1f} (events)")
+
+    # deviation FP32 vs FP64 and error vs the analytic rotated field after W+K steps
+    dev = float(np.abs(u32 - u64).max() / np.abs(u64).max())
+    t_end = (W + K) * dt
+    xr = rotate(xyz, omega, -np.linalg.norm(omega) * t_end)
+    exact = np.zeros_like(u0)
+    for f in range(F):
+        for l, m, c in spec[f]:
+            exact[:, f] += c * real_sph_harm(int(l), int(m), xr)
+    err64 = float(np.abs(u64 - exact).max() / np.abs(exact).max())
+    err32 = float(np.abs(u32 - exact).max() / np.abs(exact).max())
+    log(f"rel. deviation FP32 vs FP64 {dev:.2e}; max rel error vs analytic: FP64 {err64:.2e}, FP32 {err32:.2e}")
+
+    # ---- dominant kernel: the fused stage kernel, timed per launch ---------------------------
+    R_L = 50
+    stage_ms = {}
+    for name, rk, s_w, s_f in (("fp32", rk32, 4, 4), ("fp64", rk64, 8, 8)):
+        rk.set_state(d_u0)
+        for s in (1, 2, 3, 4):
+            rk.stage(s, dt)
+        ctx.sync()
+        per = []
+        for s in (1, 2, 3, 4):
+            ctx.timer_start()
+            for _ in range(R_L):
+                rk.stage(s, dt)
+            per.append(ctx.timer_stop() / R_L)
+        stage_ms[name] = per
+    bytes32 = [bytes_per_stage(N, n, 4, F, 4, 4, 3, s) for s in (1, 2, 3, 4)]
+    bytes64 = [bytes_per_stage(N, n, 4, F, 8, 8, 3, s) for s in (1, 2, 3, 4)]
+    avg_ms32 = float(np.mean(stage_ms["fp32"]))
+    avg_b32 = float(np.mean(bytes32))
+    ach32 = avg_b32 / (avg_ms32 * 1e-3) / 1e9
+    ach64 = float(np.mean(bytes64)) / (float(np.mean(stage_ms["fp64"])) * 1e-3) / 1e9
+    log(f"stage kernel avg: fp32 {avg_ms32 * 1e3:.1f} us ({ach32:.0f} GB/s), "
+        f"fp64 {np.mean(stage_ms['fp64']) * 1e3:.1f} us ({ach64:.0f} GB/s)")
+
+    # ---- SpMV (1 operator, 1 field) ------------------------------------------------------------
+    spmv = {}
+    for name, ell, dt_np, ft, s_w in (("fp64", ell_sp64, np.float64, R.F64, 8), ("fp32", ell_sp32, np.float32, R.F32, 4)):
+        d_x = ctx.to_device(u0[:, 0].astype(dt_np))
+        d_y = ctx.alloc(N, dt_np)
+        for _ in range(5):
+            ell.spmv(0, d_x, d_y, ft, R.F64)
+        ctx.timer_start()
+        for _ in range(R_L):
+            ell.spmv(0, d_x, d_y, ft, R.F64)
+        ms = ctx.timer_stop() / R_L
+        b = N * n * (s_w + 4) + N * 2 * np.dtype(dt_np).itemsize
+        spmv[name] = {"us": ms * 1e3, "GB_s": b / (ms * 1e-3) / 1e9, "bytes": b}
+    log(f"SpMV: {spmv}")
+
+    # ---- CPU baseline: the oracle's RK4 step on the same workload (all host cores) --------------
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+
+        nthreads = len(os.sched_getaffinity(0))
+        w_host = [v.download() for v in w_views]
+        coefs_h = [-vel[:, 0].copy(), -vel[:, 1].copy(), -vel[:, 2].copy(), None]
+        t = time.perf_counter()
+        O.rk4_step(st, w_host, coefs_h, u0, dt, n_steps=args.cpu_steps, n_threads=nthreads)
+        tc = time.perf_counter() - t
+        cpu = {"value": args.cpu_steps / tc, "unit": "RK4 steps/s", "cores": nthreads, "kind": "port",
+               "sample": f"{args.cpu_steps} FP64 RK4 step(s) of the same N={N}, n={n}, 4-operator, {F}-field "
+                         f"workload with the C++ restatement (oracle/oracle.cpp), OpenMP over rows"}
+        log(f"CPU baseline: {cpu}")
+
+    value = K / wall32
+    out = {
+        "metric": "RK4 steps/s + SpMV HBM GB/s at N=1M,n=50; FP32 vs FP64 at stated error",
+        "value": round(value, 2),
+        "unit": "RK4 steps/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": round(1e3 * wall32 / K, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 storage / f64 accumulate+state",
+        "data": "synthetic (seeded sphere nodes, spherical-harmonic fields)",
+        "config": {"workload": f"fused RK4 step: N={N}, n={n} kd-tree stencils, 4 operators (X,Y,Z,HV Delta^4) x "
+                               f"{F} fields, FP32-stored weights+fields, FP64 accumulation and state, hipGraph",
+                   "global_batch": N, "seq_len": n, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(ach32, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(ach32 / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "rk4_stage_kernel<float,float,double,4,*>",
+                     "bytes_per_launch": int(avg_b32), "avg_launch_us": round(avg_ms32 * 1e3, 2)},
+        "cpu_baseline": cpu,
+        "extra": {
+            "fp64_steps_per_s": round(K / wall64, 2),
+            "fp32_over_fp64": round(wall64 / wall32, 3),
+            "fp32_rel_deviation_from_fp64": dev,
+            "rel_error_vs_analytic": {"fp64": err64, "fp32": err32, "t": t_end},
+            "step_roofline_frac": {"fp32": round(sum(bytes32) / (wall32 / K) / 1e9 / HBM_PEAK_GBS, 4),
+                                   "fp64": round(sum(bytes64) / (wall64 / K) / 1e9 / HBM_PEAK_GBS, 4)},
+            "stage_us": {k: [round(x * 1e3, 2) for x in v] for k, v in stage_ms.items()},
+            "stage_GB_s_fp64": round(ach64, 1),
+            "spmv": spmv,
+            "weights_s": round(t_weights, 4),
+            "device": info,
+        },
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -108,7 +108,9 @@ int rbfx_nodes_fibonacci(int32_t n_nodes, double jitter, uint64_t seed, double* 
  * dertypes: RBFX_* mask; one output column per set bit in ascending bit order.
  * dim: dimension of LAPL and of the HV recurrence (RBFFD(..., dim_num, ...)).
  * poly_deg: -1 = no polynomial, 0 = the reference's single constant monomial (ONE_MONOMIAL,
- *   rbffd.cpp:1), 1..3 = full monomial basis in centred coordinates (x - x_centre).
+ *   rbffd.cpp:1), 1..3 = full monomial basis in centred coordinates (x - x_centre). A stencil
+ *   with fewer nodes than monomials uses the largest degree that fits (n = 17 with degree 3
+ *   uses degree 2), since [A P; P' 0] is singular otherwise.
  * hv_k, hv_scale: Delta^k order and the column scale -gamma*N^-k (RBFFD::getHVScalar,
  *   rbffd.h:227-238); hv_scale multiplies the HV column only.
  * Output (device): d_w[t * nnz + d_st_off[s] + j] for type column t, node j < n_s
@@ -172,8 +174,28 @@ int rbfx_rk4_set_state(rbfx_rk4* rk, const double* d_u);
 int rbfx_rk4_get_state(rbfx_rk4* rk, double* d_u);
 /* Advances n_steps steps of size dt (TimeDependentPDE::advance(RK4, dt) repeated). */
 int rbfx_rk4_step(rbfx_rk4* rk, double dt, int32_t n_steps);
-/* One stage kernel launch (stage 1..4) for profiling the dominant kernel in isolation. */
+/* One stage kernel launch (stage 1..4) for profiling the dominant kernel in isolation.
+ * It updates the internal state like the stage would inside a step. */
 int rbfx_rk4_stage(rbfx_rk4* rk, int32_t stage, double dt);
+
+/* ---- launch-shape tuning (fused kernels; kind 0 = SpMM, 1 = RK4 stage) ----------------- */
+/* Fixes fields-per-thread (1,2,4,8) and block size (64..256, multiple of 64) for this operator
+ * set's row count, width, operator count, field count and precisions. rbfx_rk4_step picks
+ * the shape up when it (re)captures its graph. */
+int rbfx_tune_set(rbfx_ell* ell, int32_t kind, int32_t n_fields, int32_t field_type, int32_t accum, int32_t fpt,
+                  int32_t block);
+int rbfx_tune_get(rbfx_ell* ell, int32_t kind, int32_t n_fields, int32_t field_type, int32_t accum, int32_t* fpt,
+                  int32_t* block);
+int rbfx_tune_clear(void);
+
+/* ---- cosine bell pointwise terms (src/pdes/hyperbolic/cosine_bell.cpp:23-70) ---------- */
+/* c_lambda[i] = -(u_vel/a), c_theta[i] = -(v_vel/a) at node i (device arrays), for use as the
+ * LAMBDA / THETA coefficients of rbfx_rk4_desc (the HV coefficient is 1). */
+int rbfx_cosine_bell_coefs(rbfx_ctx* ctx, int32_t n, const double* d_xyz, double a, double alpha, double t_rev,
+                           double* d_c_lambda, double* d_c_theta);
+/* Bell height (ExactAdvection, tests/experiments/cosine_bell_vcl/exact_advection.h:37-54) at
+ * each node, written to d_h[i*stride]. */
+int rbfx_cosine_bell_init(rbfx_ctx* ctx, int32_t n, const double* d_xyz, double a, double* d_h, int32_t stride);
 
 #ifdef __cplusplus
 }

@@ -255,6 +255,15 @@ int orc_num_monomials(int poly_deg) {
     return (poly_deg + 1) * (poly_deg + 2) * (poly_deg + 3) / 6;
 }
 
+// Degree actually used for a stencil of n nodes: the largest d <= poly_deg whose monomial
+// count does not exceed n (more monomials than nodes make [A P; P' 0] singular, e.g. degree 3
+// needs 20 nodes, so an n = 17 stencil falls back to degree 2). This build's rule.
+int orc_effective_degree(int poly_deg, int n) {
+    int d = poly_deg;
+    while (d > 0 && orc_num_monomials(d) > n) d--;
+    return d;
+}
+
 // getStencilLHS + distanceMatrix (rbffd.cpp:129-191, 888-969) and getStencilMultiRHS
 // (rbffd.cpp:106-125). Row-major M[r*m + c], R[r*T + t].
 int orc_stencil_system(const double* xyz, const int32_t* st, int32_t n, double eps, uint32_t dertypes, int32_t dim,
@@ -341,7 +350,6 @@ int orc_weights(int32_t n_st, const int32_t* st_off, const int32_t* st_idx, cons
                 uint32_t dertypes, int32_t dim, int32_t poly_deg, int32_t hv_k, double hv_scale, double* w,
                 double* diag, int32_t n_threads) {
     int T = num_types(dertypes);
-    int np = orc_num_monomials(poly_deg);
     size_t nnz = (size_t)st_off[n_st];
     int n_sing = 0;
 #ifdef _OPENMP
@@ -355,10 +363,11 @@ int orc_weights(int32_t n_st, const int32_t* st_off, const int32_t* st_idx, cons
 #endif
         for (int s = 0; s < n_st; s++) {
             int n = st_off[s + 1] - st_off[s];
-            int m = n + np;
+            int deg = orc_effective_degree(poly_deg, n);
+            int m = n + orc_num_monomials(deg);
             M.resize((size_t)m * m);
             R.resize((size_t)m * T);
-            orc_stencil_system(xyz, st_idx + st_off[s], n, eps[s], dertypes, dim, poly_deg, hv_k, M.data(), R.data());
+            orc_stencil_system(xyz, st_idx + st_off[s], n, eps[s], dertypes, dim, deg, hv_k, M.data(), R.data());
             if (diag) { M0 = M; R0 = R; }
             int info = orc_lu_solve(m, T, M.data(), R.data());
             if (info) n_sing++;

@@ -30,6 +30,8 @@ double orc_hv_pk(double eps, int dim, int k, double r2);
 
 /* Number of polynomial terms for poly_deg in {-1,0,1,2,3}. */
 int orc_num_monomials(int poly_deg);
+/* Largest degree <= poly_deg with at most n monomials (used per stencil by orc_weights). */
+int orc_effective_degree(int poly_deg, int n);
 
 /* Builds the saddle-point system of one stencil (row-major m x m LHS, m x T RHS). */
 int orc_stencil_system(const double* xyz, const int32_t* st, int32_t n, double eps, uint32_t dertypes,

@@ -1,0 +1,209 @@
+"""ctypes binding of liboracle.so, the CPU restatement of the reference hot path.
+
+TEST INFRASTRUCTURE: imported only by tests/, bench.py's cpu_baseline leg and
+__graft_entry__.smoke(), where it is the checker. The product never imports it.
+"""
+import ctypes as C
+import os
+import struct
+import subprocess
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+REF_KNN = os.path.join(HERE, "_ref", "ref_knn")
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            raise OSError(f"{LIB} missing: run make -C oracle")
+        L = C.CDLL(LIB)
+        d, i32, u32, vp = C.c_double, C.c_int32, C.c_uint32, C.c_void_p
+        for name in ("orc_rbf_eval", "orc_rbf_xderiv", "orc_rbf_rderiv_over_r", "orc_rbf_rderiv2"):
+            getattr(L, name).restype = d
+            getattr(L, name).argtypes = [d, d, d, d]
+        L.orc_rbf_lapl.restype = d
+        L.orc_rbf_lapl.argtypes = [d, C.c_int, d, d, d]
+        L.orc_rbf_hv.restype = d
+        L.orc_rbf_hv.argtypes = [d, C.c_int, C.c_int, d, d, d]
+        L.orc_hv_pk.restype = d
+        L.orc_hv_pk.argtypes = [d, C.c_int, C.c_int, d]
+        L.orc_num_monomials.restype = C.c_int
+        L.orc_num_monomials.argtypes = [C.c_int]
+        L.orc_effective_degree.restype = C.c_int
+        L.orc_effective_degree.argtypes = [C.c_int, C.c_int]
+        L.orc_stencil_system.restype = C.c_int
+        L.orc_stencil_system.argtypes = [vp, vp, i32, d, u32, i32, i32, i32, vp, vp]
+        L.orc_lu_solve.restype = C.c_int
+        L.orc_lu_solve.argtypes = [i32, i32, vp, vp]
+        L.orc_weights.restype = C.c_int
+        L.orc_weights.argtypes = [i32, vp, vp, vp, vp, u32, i32, i32, i32, d, vp, vp, i32]
+        L.orc_apply.restype = None
+        L.orc_apply.argtypes = [i32, vp, vp, vp, vp, vp]
+        L.orc_spmm.restype = None
+        L.orc_spmm.argtypes = [i32, i32, vp, i32, vp, i32, vp, vp, i32]
+        L.orc_rk4_step.restype = None
+        L.orc_rk4_step.argtypes = [i32, i32, vp, i32, vp, vp, i32, d, vp, i32]
+        L.orc_cart2sph.restype = None
+        L.orc_cart2sph.argtypes = [d, d, d, C.POINTER(d), C.POINTER(d), C.POINTER(d)]
+        L.orc_cosine_bell_coefs.restype = None
+        L.orc_cosine_bell_coefs.argtypes = [i32, vp, d, d, d, vp, vp]
+        L.orc_cosine_bell_exact.restype = d
+        L.orc_cosine_bell_exact.argtypes = [d, d, d, d]
+        L.orc_cosine_bell_rhs.restype = None
+        L.orc_cosine_bell_rhs.argtypes = [i32, vp, vp, vp, vp, vp, vp, d, d, d, i32, vp, vp]
+        L.orc_sol_norms.restype = None
+        L.orc_sol_norms.argtypes = [i32, vp, vp, vp]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def ntypes(mask):
+    return bin(mask & 0xFFFF).count("1")
+
+
+def effective_degree(poly_deg, n):
+    return lib().orc_effective_degree(poly_deg, n)
+
+
+def stencil_system(xyz, st, eps, dertypes, dim=3, poly_deg=0, hv_k=0):
+    xyz = np.ascontiguousarray(xyz, np.float64)
+    st = np.ascontiguousarray(st, np.int32)
+    n = st.size
+    m = n + lib().orc_num_monomials(poly_deg)
+    T = ntypes(dertypes)
+    M = np.zeros((m, m))
+    Rr = np.zeros((m, T))
+    lib().orc_stencil_system(_p(xyz), _p(st), n, eps, dertypes, dim, poly_deg, hv_k, _p(M), _p(Rr))
+    return M, Rr
+
+
+def lu_solve(M, Rr):
+    A = np.array(M, np.float64, order="C")
+    B = np.array(Rr, np.float64, order="C")
+    if B.ndim == 1:
+        B = B[:, None].copy()
+    info = lib().orc_lu_solve(A.shape[0], B.shape[1], _p(A), _p(B))
+    return B, info
+
+
+def weights(st_off, st_idx, xyz, eps, dertypes, dim=3, poly_deg=0, hv_k=0, hv_scale=1.0, diag=False, n_threads=0):
+    st_off = np.ascontiguousarray(st_off, np.int32)
+    st_idx = np.ascontiguousarray(st_idx, np.int32)
+    xyz = np.ascontiguousarray(xyz, np.float64)
+    n_st = st_off.size - 1
+    e = np.asarray(eps, np.float64)
+    if e.ndim == 0:
+        e = np.full(n_st, float(e))
+    e = np.ascontiguousarray(e)
+    T = ntypes(dertypes)
+    w = np.empty((T, st_idx.size))
+    dg = np.empty((n_st, 2)) if diag else None
+    ns = lib().orc_weights(n_st, _p(st_off), _p(st_idx), _p(xyz), _p(e), dertypes, dim, poly_deg, hv_k, hv_scale,
+                           _p(w), _p(dg) if diag else None, n_threads)
+    return w, dg, ns
+
+
+def apply(st_off, st_idx, w, u):
+    st_off = np.ascontiguousarray(st_off, np.int32)
+    st_idx = np.ascontiguousarray(st_idx, np.int32)
+    w = np.ascontiguousarray(w, np.float64)
+    u = np.ascontiguousarray(u, np.float64)
+    out = np.empty(st_off.size - 1)
+    lib().orc_apply(out.size, _p(st_off), _p(st_idx), _p(w), _p(u), _p(out))
+    return out
+
+
+def spmm(idx, ws, u, n_threads=0):
+    idx = np.ascontiguousarray(idx, np.int32)
+    n_rows, n = idx.shape
+    W = np.ascontiguousarray(np.stack([np.asarray(w, np.float64).reshape(n_rows, n) for w in ws]))
+    u = np.ascontiguousarray(u, np.float64)
+    F = 1 if u.ndim == 1 else u.shape[1]
+    out = np.empty((len(ws), n_rows, F))
+    lib().orc_spmm(n_rows, n, _p(idx), len(ws), _p(W), F, _p(u), _p(out), n_threads)
+    return out
+
+
+def rk4_step(idx, ws, coefs, u, dt, n_steps=1, n_threads=0):
+    idx = np.ascontiguousarray(idx, np.int32)
+    n_rows, n = idx.shape
+    W = np.ascontiguousarray(np.stack([np.asarray(w, np.float64).reshape(n_rows, n) for w in ws]))
+    u = np.array(u, np.float64, order="C")
+    F = 1 if u.ndim == 1 else u.shape[1]
+    keep = [None if c is None else np.ascontiguousarray(c, np.float64) for c in (coefs or [None] * len(ws))]
+    arr = (C.c_void_p * len(ws))(*[None if c is None else c.ctypes.data for c in keep])
+    for _ in range(n_steps):
+        lib().orc_rk4_step(n_rows, n, _p(idx), len(ws), _p(W), arr, F, dt, _p(u), n_threads)
+    return u
+
+
+def cart2sph(x, y, z):
+    t, p, r = C.c_double(), C.c_double(), C.c_double()
+    lib().orc_cart2sph(x, y, z, C.byref(t), C.byref(p), C.byref(r))
+    return t.value, p.value, r.value
+
+
+def cosine_bell_coefs(xyz, a=1.0, alpha=np.pi / 2, t_rev=1036800.0):
+    xyz = np.ascontiguousarray(xyz, np.float64)
+    cl = np.empty(xyz.shape[0])
+    ct = np.empty(xyz.shape[0])
+    lib().orc_cosine_bell_coefs(xyz.shape[0], _p(xyz), a, alpha, t_rev, _p(cl), _p(ct))
+    return cl, ct
+
+
+def cosine_bell_exact(xyz, a=1.0):
+    return np.array([lib().orc_cosine_bell_exact(p[0], p[1], p[2], a) for p in np.asarray(xyz, np.float64)])
+
+
+def cosine_bell_rhs(st_off, st_idx, w_lambda, w_theta, w_hv, xyz, u, a=1.0, alpha=np.pi / 2, t_rev=1036800.0, use_hv=1):
+    args = [np.ascontiguousarray(x, t) for x, t in ((st_off, np.int32), (st_idx, np.int32), (w_lambda, np.float64),
+                                                    (w_theta, np.float64), (w_hv, np.float64), (xyz, np.float64),
+                                                    (u, np.float64))]
+    f = np.empty(args[0].size - 1)
+    lib().orc_cosine_bell_rhs(f.size, _p(args[0]), _p(args[1]), _p(args[2]), _p(args[3]), _p(args[4]), _p(args[5]), a,
+                              alpha, t_rev, use_hv, _p(args[6]), _p(f))
+    return f
+
+
+def sol_norms(approx, exact):
+    a = np.ascontiguousarray(approx, np.float64)
+    e = np.ascontiguousarray(exact, np.float64)
+    out = np.empty(6)
+    lib().orc_sol_norms(a.size, _p(a), _p(e), _p(out))
+    return dict(zip(("l1", "l1rel", "l2", "l2rel", "linf", "linfrel"), out))
+
+
+def ref_knn(xyz, k):
+    """Stencils from the reference's own kd-tree (oracle/_ref/ref_knn, built from its sources)."""
+    if not os.path.exists(REF_KNN):
+        raise FileNotFoundError(REF_KNN)
+    xyz = np.ascontiguousarray(xyz, np.float64)
+    with tempfile.TemporaryDirectory() as td:
+        fn, fo = os.path.join(td, "n.bin"), os.path.join(td, "o.bin")
+        with open(fn, "wb") as f:
+            f.write(struct.pack("<q", xyz.shape[0]))
+            f.write(xyz.tobytes())
+        subprocess.run([REF_KNN, fn, str(k), fo], check=True, stdout=subprocess.DEVNULL)
+        d = np.fromfile(fo, dtype=np.int32)
+    return d[2:].reshape(d[0], d[1])
+
+
+def knn_brute(xyz, k):
+    """Exact k nearest by full sort of squared distances (ties by index)."""
+    xyz = np.asarray(xyz, np.float64)
+    out = np.empty((xyz.shape[0], k), np.int32)
+    for i in range(xyz.shape[0]):
+        d = ((xyz - xyz[i]) ** 2).sum(1)
+        out[i] = np.lexsort((np.arange(d.size), d))[:k]
+    return out

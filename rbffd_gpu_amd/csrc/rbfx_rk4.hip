@@ -275,8 +275,13 @@ int rbfx_rk4_step(rbfx_rk4* rk, double dt, int32_t n_steps) {
     if (!rk || n_steps < 0) return RBFX_E_INVALID;
     if (!rk->have_state) return rbfx_fail(rk->ctx, RBFX_E_STATE, "rk4_step: call rbfx_rk4_set_state first");
     RBFX_HIP_CHECK(rk->ctx, hipSetDevice(rk->ctx->device));
+    EllTune t;
+    int prc = rbfx_ell_pick(rk->ell, rk->F, rk->field_storage, rk->accum, 1, &t);
+    if (prc) return prc;
+    bool shape_changed = (t.fpt != rk->tune.fpt || t.block != rk->tune.block);
+    rk->tune = t;
     if (rk->use_graph) {
-        if (!rk->exec || rk->graph_dt != dt) {
+        if (!rk->exec || rk->graph_dt != dt || shape_changed) {
             int rc = capture_step(rk, dt);
             if (rc) return rc;
         }
@@ -294,6 +299,8 @@ int rbfx_rk4_step(rbfx_rk4* rk, double dt, int32_t n_steps) {
 int rbfx_rk4_stage(rbfx_rk4* rk, int32_t stage, double dt) {
     if (!rk || stage < 1 || stage > 4) return RBFX_E_INVALID;
     if (!rk->have_state) return rbfx_fail(rk->ctx, RBFX_E_STATE, "rk4_stage: call rbfx_rk4_set_state first");
+    int rc = rbfx_ell_pick(rk->ell, rk->F, rk->field_storage, rk->accum, 1, &rk->tune);
+    if (rc) return rc;
     return launch_stage(rk, stage, dt);
 }
 

@@ -25,7 +25,7 @@ namespace {
 constexpr int WBLOCK = 256;
 constexpr int MAX_M = 128;
 constexpr int MAX_T = 16;
-constexpr size_t LDS_LIMIT = 160 * 1024;
+constexpr size_t STATIC_LDS_RESERVE = 1024;  // static __shared__ scratch of the kernels (< 1 KB)
 
 struct Mono3 {
     int8_t a, b, c;
@@ -38,6 +38,14 @@ __constant__ Mono3 c_mono[20] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {2,
                                  {1, 1, 1}, {1, 0, 2}, {0, 3, 0}, {0, 2, 1}, {0, 1, 2}, {0, 0, 3}};
 
 int num_monomials(int deg) { return deg < 0 ? 0 : (deg + 1) * (deg + 2) * (deg + 3) / 6; }
+
+// Degree used for a stencil of n nodes: the largest d <= poly_deg with at most n monomials
+// (more monomials than nodes make [A P; P' 0] singular; degree 3 needs 20 nodes).
+int effective_degree(int poly_deg, int n) {
+    int d = poly_deg;
+    while (d > 0 && num_monomials(d) > n) d--;
+    return d;
+}
 
 struct WParams {
     const int32_t* st_off;
@@ -677,7 +685,8 @@ extern "C" int rbfx_weights(rbfx_ctx* ctx, int32_t n_stencils, const int32_t* d_
         if (n < 1) return rbfx_fail(ctx, RBFX_E_INVALID, "weights: stencil %d has size %d", s, n);
         max_n = std::max(max_n, n);
     }
-    if (max_n + P.np > MAX_M) return rbfx_fail(ctx, RBFX_E_TOO_LARGE, "weights: system size %d exceeds %d", max_n + P.np, MAX_M);
+    if (max_n + num_monomials(effective_degree(poly_deg, max_n)) > MAX_M)
+        return rbfx_fail(ctx, RBFX_E_TOO_LARGE, "weights: system size %d exceeds %d", max_n + num_monomials(effective_degree(poly_deg, max_n)), MAX_M);
     P.nnz = off[n_stencils];
     {
         std::vector<int32_t> idx(P.nnz);
@@ -718,6 +727,10 @@ extern "C" int rbfx_weights(rbfx_ctx* ctx, int32_t n_stencils, const int32_t* d_
         return rbfx_fail(ctx, RBFX_E_HIP, "weights: %s", hipGetErrorString(e));
     }
     P.fail_flags = d_flags;
+    int lds_attr = 0;
+    if (hipDeviceGetAttribute(&lds_attr, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device) != hipSuccess || lds_attr <= 0)
+        lds_attr = 64 * 1024;
+    const size_t lds_limit = (size_t)lds_attr;
 
     auto launch_classes = [&](bool ir, const int32_t* ord, const std::vector<int>& sizes_count,
                               const std::vector<int>& sizes_start) -> int {
@@ -727,10 +740,11 @@ extern "C" int rbfx_weights(rbfx_ctx* ctx, int32_t n_stencils, const int32_t* d_
             WParams Q = P;
             Q.order = ord + sizes_start[n];
             Q.count = c;
-            Q.m = n + P.np;
+            Q.np = num_monomials(effective_degree(poly_deg, n));
+            Q.m = n + Q.np;
             Q.fp32 = ir ? 1 : 0;
             size_t lds = ir ? lds_bytes<float>(Q.m, n, T, true) : lds_bytes<double>(Q.m, n, T, false);
-            if (lds > LDS_LIMIT) return rbfx_fail(ctx, RBFX_E_TOO_LARGE, "weights: system %d x %d with %d RHS needs %zu B of LDS", Q.m, Q.m, T, lds);
+            if (lds + STATIC_LDS_RESERVE > lds_limit) return rbfx_fail(ctx, RBFX_E_TOO_LARGE, "weights: system %d x %d with %d RHS needs %zu B of LDS", Q.m, Q.m, T, lds);
             if (ir)
                 hipLaunchKernelGGL(weights_f32ir_kernel, dim3(c), dim3(WBLOCK), lds, st, Q);
             else

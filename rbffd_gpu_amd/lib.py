@@ -80,6 +80,8 @@ def load_library(path=None):
         "rbfx_tune_set": (C.c_int, [vp, i32, i32, i32, i32, i32, i32]),
         "rbfx_tune_get": (C.c_int, [vp, i32, i32, i32, i32, C.POINTER(i32), C.POINTER(i32)]),
         "rbfx_tune_clear": (C.c_int, []),
+        "rbfx_cosine_bell_coefs": (C.c_int, [vp, i32, vp, f64, f64, f64, vp, vp]),
+        "rbfx_cosine_bell_init": (C.c_int, [vp, i32, vp, f64, vp, i32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -194,6 +196,20 @@ class Context:
             return d_w, d_diag
         return d_w.download(), (d_diag.download() if d_diag else None)
 
+    def cosine_bell_coefs(self, d_xyz, a=1.0, alpha=3.141592653589793 / 2, t_rev=1036800.0):
+        n = d_xyz.shape[0]
+        cl, ct = self.alloc(n, np.float64), self.alloc(n, np.float64)
+        self.check(RBFX.rbfx_cosine_bell_coefs(self.h, n, d_xyz.ptr, a, alpha, t_rev, cl.ptr, ct.ptr), "cosine_bell_coefs")
+        return cl, ct
+
+    def cosine_bell_init(self, d_xyz, a=1.0, n_fields=1):
+        n = d_xyz.shape[0]
+        h = self.alloc((n, n_fields), np.float64)
+        for f in range(n_fields):
+            self.check(RBFX.rbfx_cosine_bell_init(self.h, n, d_xyz.ptr, a, C.c_void_p(h.ptr.value + 8 * f), n_fields),
+                       "cosine_bell_init")
+        return h
+
     def close(self):
         for b in self._buffers:
             b.free()
@@ -240,15 +256,40 @@ class DeviceArray:
             RBFX.rbfx_dev_free(self.ctx.h, self._p)
             self._p = C.c_void_p()
 
+    def view(self, offset_elems, shape):
+        """Non-owning view of part of this buffer (e.g. one operator's weights)."""
+        return DeviceView(self, offset_elems, shape)
+
+
+class DeviceView:
+    def __init__(self, base, offset_elems, shape):
+        self.ctx = base.ctx
+        self.base = base
+        self.dtype = base.dtype
+        self.shape = tuple(shape)
+        self.size = int(np.prod(self.shape))
+        self.nbytes = self.size * self.dtype.itemsize
+        assert offset_elems + self.size <= base.size
+        self._p = C.c_void_p(base.ptr.value + offset_elems * self.dtype.itemsize)
+
+    @property
+    def ptr(self):
+        return self._p
+
+    def download(self):
+        out = np.empty(self.shape, dtype=self.dtype)
+        self.ctx.check(RBFX.rbfx_d2h(self.ctx.h, _ptr(out), self._p, self.nbytes), "d2h")
+        return out
+
 
 class EllOperators:
     """Fused ELL operator set: n_ops operators sharing one (n_rows, width) index array."""
 
     def __init__(self, ctx, idx, weights, n_cols=None, storage=F64, types=None):
         self.ctx = ctx
-        d_idx = idx if isinstance(idx, DeviceArray) else ctx.to_device(np.asarray(idx, np.int32))
+        d_idx = idx if isinstance(idx, (DeviceArray, DeviceView)) else ctx.to_device(np.asarray(idx, np.int32))
         n_rows, width = d_idx.shape
-        d_ws = [w if isinstance(w, DeviceArray) else ctx.to_device(np.asarray(w, np.float64).reshape(n_rows, width))
+        d_ws = [w if isinstance(w, (DeviceArray, DeviceView)) else ctx.to_device(np.asarray(w, np.float64).reshape(n_rows, width))
                 for w in weights]
         n_ops = len(d_ws)
         ptrs = (C.c_void_p * n_ops)(*[w.ptr.value for w in d_ws])
@@ -292,7 +333,7 @@ class RK4:
         for o in range(ell.n_ops):
             c = None if coefs is None else coefs[o]
             if c is not None:
-                dc = c if isinstance(c, DeviceArray) else ctx.to_device(np.asarray(c, np.float64))
+                dc = c if isinstance(c, (DeviceArray, DeviceView)) else ctx.to_device(np.asarray(c, np.float64))
                 self._coefs.append(dc)
                 d.d_coef[o] = dc.ptr.value
         d.use_graph = 1 if use_graph else 0
@@ -303,7 +344,7 @@ class RK4:
         self.n_fields = n_fields
 
     def set_state(self, u):
-        d = u if isinstance(u, DeviceArray) else self.ctx.to_device(np.asarray(u, np.float64))
+        d = u if isinstance(u, (DeviceArray, DeviceView)) else self.ctx.to_device(np.asarray(u, np.float64))
         self.ctx.check(RBFX.rbfx_rk4_set_state(self.h, d.ptr), "rk4_set_state")
         self._state_in = d
 

@@ -1,0 +1,145 @@
+"""GPU parity of derivative application (SpMV / fused SpMM) against the CPU restatement.
+
+Parity metric for FP64 (cancellation-aware, derivative weights sum to zero):
+    |y_gpu - y_cpu| / sum_j |w_ij u_j|  <= 1e-12  per row.
+FP32-stored weights/fields are compared against the FP64 result with the same
+normalisation; the bound is the FP32 rounding of the inputs (a few u32 = 6e-8).
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+import rbffd_gpu_amd as R
+from conftest import rel_apply_err
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-12
+TOL32 = 1e-6
+
+
+def _absum(idx, w, u):
+    """sum_j |w_ij u_idx(ij)| per row (and field)."""
+    return np.einsum("ij,ijf->if", np.abs(w), np.abs(u[idx])) if u.ndim == 2 else (np.abs(w) * np.abs(u[idx])).sum(1)
+
+
+def test_spmv_c1_reference_weights(ctx, c1):
+    """Config 1: oracle weights (X,Y,Z,LSFC; deg 2) applied on the GPU, one operator at a time."""
+    print("case: spmv c1 N=4096 n=31", flush=True)
+    xyz, st = c1["xyz"], c1["st"]
+    off, idx = R.csr_from_fixed(st)
+    mask = R.X | R.Y | R.Z | R.LSFC
+    w, _, ns = O.weights(off, idx, xyz, 3.5, mask, dim=3, poly_deg=2)
+    assert ns == 0
+    x, y, z = xyz.T
+    f = np.exp(x) * np.sin(2 * y) + z ** 3
+    d_f = ctx.to_device(f)
+    d_out = ctx.alloc(4096, np.float64)
+    for t in range(4):
+        ref = O.apply(off, idx, w[t], f)
+        ell = R.EllOperators(ctx, st, [w[t]], storage=R.F64)
+        ell.spmv(0, d_f, d_out)
+        got = d_out.download()
+        ell.close()
+        err = rel_apply_err(got, ref, _absum(st, w[t].reshape(st.shape), f))
+        print(f"  type {t}: normalised err {err:.2e}")
+        assert err <= TOL64
+
+
+def test_spmv_interleaved_operator(ctx, c1):
+    """Single operator out of a 4-operator interleaved set (strided weights)."""
+    print("case: spmv strided c1", flush=True)
+    xyz, st = c1["xyz"], c1["st"]
+    rng = np.random.default_rng(3)
+    ws = [rng.standard_normal(st.shape) for _ in range(4)]
+    u = rng.standard_normal(4096)
+    ell = R.EllOperators(ctx, st, ws, storage=R.F64)
+    d_u = ctx.to_device(u)
+    d_o = ctx.alloc(4096, np.float64)
+    for o in range(4):
+        ell.spmv(o, d_u, d_o)
+        off, idx = R.csr_from_fixed(st)
+        ref = O.apply(off, idx, ws[o].reshape(-1), u)
+        assert rel_apply_err(d_o.download(), ref, _absum(st, ws[o], u)) <= TOL64
+    ell.close()
+
+
+def test_apply_exact_integer_kat(ctx):
+    """Small-integer weights and fields: every partial sum is exact in FP64 and FP32."""
+    print("case: integer KAT", flush=True)
+    rng = np.random.default_rng(5)
+    n_rows, width = 1000, 17
+    idx = rng.integers(0, n_rows, size=(n_rows, width)).astype(np.int32)
+    idx[:, 0] = np.arange(n_rows)
+    w = rng.integers(-8, 9, size=(n_rows, width)).astype(np.float64)
+    u = rng.integers(-16, 17, size=n_rows).astype(np.float64)
+    exact = (w * u[idx]).sum(1)
+    for storage, ft, acc in ((R.F64, R.F64, R.F64), (R.F32, R.F32, R.F64), (R.F32, R.F32, R.F32)):
+        ell = R.EllOperators(ctx, idx, [w], storage=storage)
+        dt = np.float64 if ft == R.F64 else np.float32
+        d_u = ctx.to_device(u.astype(dt))
+        d_o = ctx.alloc(n_rows, dt)
+        ell.spmv(0, d_u, d_o, ft, acc)
+        assert np.array_equal(d_o.download().astype(np.float64), exact), (storage, ft, acc)
+        ell.close()
+
+
+@pytest.mark.parametrize("n_ops,F", [(1, 1), (3, 1), (4, 16), (4, 8), (2, 6), (4, 3)])
+def test_spmm_fused(ctx, n_ops, F):
+    """Fused multi-operator multi-field SpMM, FP64 and FP32-stored, vs the CPU restatement."""
+    print(f"case: spmm n_ops={n_ops} F={F}", flush=True)
+    rng = np.random.default_rng(11 + n_ops * 7 + F)
+    n_rows, width = 5000, 50
+    idx = rng.integers(0, n_rows, size=(n_rows, width)).astype(np.int32)
+    idx[:, 0] = np.arange(n_rows)
+    ws = [rng.standard_normal((n_rows, width)) for _ in range(n_ops)]
+    u = rng.standard_normal((n_rows, F))
+    ref = O.spmm(idx, ws, u)
+    absum = np.stack([_absum(idx, w, u) for w in ws])
+    ell64 = R.EllOperators(ctx, idx, ws, storage=R.F64)
+    d_u = ctx.to_device(u)
+    d_o = ctx.alloc((n_ops, n_rows, F), np.float64)
+    ell64.spmm(F, d_u, d_o)
+    got = d_o.download()
+    assert rel_apply_err(got, ref, absum) <= TOL64
+    ell64.close()
+    ell32 = R.EllOperators(ctx, idx, ws, storage=R.F32)
+    d_u32 = ctx.to_device(u.astype(np.float32))
+    for acc in (R.F64, R.F32):
+        d_o32 = ctx.alloc((n_ops, n_rows, F), np.float32)
+        ell32.spmm(F, d_u32, d_o32, R.F32, acc)
+        dev = rel_apply_err(d_o32.download().astype(np.float64), ref, absum)
+        print(f"  fp32 storage, accum {'f64' if acc == R.F64 else 'f32'}: normalised deviation {dev:.2e}")
+        assert dev <= TOL32
+    ell32.close()
+
+
+def test_spmm_tuned_shapes_agree(ctx):
+    """Every launch shape the autotuner may pick gives bit-identical FP64 results."""
+    print("case: spmm launch shapes", flush=True)
+    rng = np.random.default_rng(21)
+    n_rows, width, F = 3000, 31, 16
+    idx = rng.integers(0, n_rows, size=(n_rows, width)).astype(np.int32)
+    ws = [rng.standard_normal((n_rows, width)) for _ in range(4)]
+    u = rng.standard_normal((n_rows, F))
+    ell = R.EllOperators(ctx, idx, ws, storage=R.F64)
+    d_u = ctx.to_device(u)
+    outs = []
+    for fpt in (1, 2, 4, 8):
+        for block in (64, 128, 256):
+            ell.tune_set(0, F, R.F64, R.F64, fpt, block)
+            d_o = ctx.alloc((4, n_rows, F), np.float64)
+            ell.spmm(F, d_u, d_o)
+            outs.append(d_o.download())
+    R.RBFX.rbfx_tune_clear()
+    for o in outs[1:]:
+        assert np.array_equal(o, outs[0])
+    ell.close()
+
+
+def test_ell_rejects_out_of_range_index(ctx):
+    rng = np.random.default_rng(1)
+    idx = rng.integers(0, 100, size=(100, 5)).astype(np.int32)
+    idx[7, 3] = 100
+    with pytest.raises(R.RbfxError):
+        R.EllOperators(ctx, idx, [np.ones((100, 5))])

@@ -1,0 +1,116 @@
+"""GPU batched weight solves vs the CPU restatement and vs LAPACK-solved golden systems.
+
+Weight parity cannot be elementwise at 1e-12: two partial-pivot LU orderings of a system
+with condition number kappa agree only to about kappa * u64 (kappa = 1e8..1e12 here). The
+bars are therefore
+  * normwise backward error ||R - M W|| / (||M|| ||W|| + ||R||) <= 64 * m * u64, and
+  * forward agreement |W_gpu - W_ref| / |W_ref| <= 64 * kappa * u64 (capped at 1),
+and the applied derivatives agree with the CPU path to the same forward bound.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+import rbffd_gpu_amd as R
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+U64 = 2.220446049250313e-16
+
+
+def test_weights_c1_fp64(ctx, c1):
+    print("case: weights c1 deg2 X|Y|Z|LSFC", flush=True)
+    xyz, st = c1["xyz"], c1["st"]
+    off, idx = R.csr_from_fixed(st)
+    mask = R.X | R.Y | R.Z | R.LSFC
+    w_ref, dg_ref, ns = O.weights(off, idx, xyz, 3.5, mask, dim=3, poly_deg=2, diag=True)
+    w, dg = ctx.weights(off, idx, xyz, 3.5, mask, dim=3, poly_deg=2, diag=True)
+    m = 31 + 10
+    print("  backward error gpu max %.2e  cpu max %.2e" % (dg[:, 0].max(), dg_ref[:, 0].max()))
+    assert np.all(dg[:, 3] == R.lib.PATH_F64)
+    assert dg[:, 0].max() <= 64 * m * U64
+    # forward agreement per stencil against the conditioning of its system
+    rel = np.abs(w - w_ref).reshape(4, 4096, 31).max(axis=(0, 2)) / np.abs(w_ref).reshape(4, 4096, 31).max(axis=(0, 2))
+    print("  forward rel diff max %.2e median %.2e" % (rel.max(), np.median(rel)))
+    conds = np.array([np.linalg.cond(O.stencil_system(xyz, st[i], 3.5, mask, 3, 2, 0)[0]) for i in range(0, 4096, 256)])
+    assert np.all(rel[::256] <= np.minimum(1.0, 64 * conds * U64))
+    # constant reproduction: sum_j w_j = 0 for every derivative operator
+    sums = np.abs(w.reshape(4, 4096, 31).sum(2)) / np.abs(w).reshape(4, 4096, 31).sum(2)
+    assert sums.max() < 1e-6
+
+
+def test_weights_lapack_golden(ctx):
+    """Golden systems solved by LAPACK (scipy.linalg.solve): uneven sizes 17..101, degree 0/2/3."""
+    g = np.load(os.path.join(GOLDEN, "lapack_weights.npz"))
+    for q in range(int(g["count"])):
+        pts, meta, x = g[f"s{q}_xyz"], g[f"s{q}_meta"], g[f"s{q}_x"]
+        eps, mask, dim, deg, hvk, cond = meta[0], int(meta[1]), int(meta[2]), int(meta[3]), int(meta[4]), meta[5]
+        n = pts.shape[0]
+        off = np.array([0, n], np.int32)
+        idx = np.arange(n, dtype=np.int32)
+        w, dg = ctx.weights(off, idx, pts, eps, mask, dim=dim, poly_deg=deg, hv_k=hvk, diag=True)
+        ref = x[:n].T
+        rel = np.abs(w - ref).max() / np.abs(ref).max()
+        bound = min(1.0, 64 * cond * U64)
+        assert dg[0, 0] <= 64 * (n + 20) * U64, (q, dg[0])
+        assert rel <= bound, (q, rel, cond)
+
+
+def test_weights_uneven_batch_matches_cpu(ctx):
+    """One call over stencils of sizes {17,31,50,75,101} mixed, degree 2 and 3, X/Y/Z/LAPL."""
+    print("case: uneven batch", flush=True)
+    N = 20000
+    xyz = R.nodes_fibonacci(N, 1e-3, 1)
+    rng = np.random.default_rng(1)
+    sizes = rng.choice([17, 31, 50, 75, 101], size=2000)
+    rows = rng.choice(N, size=2000, replace=False)
+    knn = {k: R.knn_kdtree(xyz, k) for k in (17, 31, 50, 75, 101)}
+    c = {17: (0.026, 0.08), 31: (0.035, 0.1), 50: (0.044, 0.14), 75: (0.051, 0.15), 101: (0.058, 0.16)}
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int32)
+    idx = np.concatenate([knn[k][r] for k, r in zip(sizes, rows)]).astype(np.int32)
+    eps = np.array([c[k][0] * np.sqrt(N) - c[k][1] for k in sizes])
+    mask = R.X | R.Y | R.Z | R.LAPL
+    for deg in (2, 3):
+        w_ref, dg_ref, ns = O.weights(off, idx, xyz, eps, mask, dim=3, poly_deg=deg, diag=True)
+        w, dg = ctx.weights(off, idx, xyz, eps, mask, dim=3, poly_deg=deg, diag=True)
+        print(f"  deg {deg}: gpu backward err max {dg[:, 0].max():.2e}, cpu {dg_ref[:, 0].max():.2e}")
+        assert dg[:, 0].max() <= 64 * 121 * U64
+        # derivatives of a smooth field through both weight sets agree to the forward bound
+        x, y, z = xyz.T
+        f = np.sin(x) * np.cos(y) + z
+        for t in range(4):
+            a = O.apply(off, idx, w[t], f)
+            b = O.apply(off, idx, w_ref[t], f)
+            scale = np.array([np.abs(w_ref[t, off[s]:off[s + 1]]).sum() for s in range(2000)]) * np.abs(f).max()
+            assert np.max(np.abs(a - b) / scale) < 1e-4
+
+
+def test_weights_f32_ir(ctx, c1):
+    """FP32 LU + FP64 refinement: converged stencils meet the FP64 backward-error bar,
+    the others are re-solved in FP64 and flagged; the weights match the FP64 path."""
+    print("case: weights fp32 + IR", flush=True)
+    xyz, st = c1["xyz"], c1["st"]
+    off, idx = R.csr_from_fixed(st)
+    mask = R.X | R.Y | R.Z | R.LSFC
+    w64, _ = ctx.weights(off, idx, xyz, 3.5, mask, dim=3, poly_deg=2, diag=True)
+    w32, dg = ctx.weights(off, idx, xyz, 3.5, mask, dim=3, poly_deg=2, diag=True, mode=R.SOLVE_F32_LU_F64_IR)
+    paths = dg[:, 3].astype(int)
+    n_ir = int((paths == R.lib.PATH_F32_IR).sum())
+    n_fb = int((paths == R.lib.PATH_F32_FALLBACK_F64).sum())
+    print(f"  converged in FP32+IR: {n_ir}, FP64 fallback: {n_fb}")
+    assert n_ir + n_fb == 4096
+    assert np.all(np.isfinite(w32))
+    assert dg[:, 0].max() <= 64 * 41 * U64
+    rel = np.abs(w32 - w64).max() / np.abs(w64).max()
+    assert rel < 1e-4
+
+
+def test_weights_singular_reported(ctx):
+    """A duplicated node makes the system singular: reported, not silently propagated."""
+    pts = np.array([[0, 0, 1], [0.1, 0, 0.99], [0.1, 0, 0.99], [0, 0.1, 0.99], [-0.1, 0, 0.99]], float)
+    off = np.array([0, 5], np.int32)
+    with pytest.raises(R.RbfxError) as e:
+        ctx.weights(off, np.arange(5, dtype=np.int32), pts, 1.0, R.X, dim=3, poly_deg=0)
+    assert e.value.code == -5
