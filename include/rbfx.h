@@ -53,6 +53,10 @@ extern "C" {
 #define RBFX_SOLVE_F64_LU 0          /* FP64 partial-pivot LU in LDS                       */
 #define RBFX_SOLVE_F32_LU_F64_IR 1   /* FP32 LU + FP64-residual refinement, FP64 fallback  */
 
+/* poly_deg flag of rbfx_weights: sphere-reduced monomial basis for nodes exactly on the unit
+ * sphere (degree 2: 9 functions, degree 3: 16; the full basis is singular there). */
+#define RBFX_POLY_SPHERE 16
+
 /* per-stencil diagnostic path flags (diag_out[4*s+3]) */
 #define RBFX_PATH_F64 0
 #define RBFX_PATH_F32_IR 1
@@ -81,6 +85,10 @@ int rbfx_h2d(rbfx_ctx* ctx, void* d_dst, const void* h_src, uint64_t bytes);
 int rbfx_d2h(rbfx_ctx* ctx, void* h_dst, const void* d_src, uint64_t bytes);
 int rbfx_d2d(rbfx_ctx* ctx, void* d_dst, const void* d_src, uint64_t bytes);
 
+/* Plain device-to-device streaming copy kernel (16-byte aligned pointers and size): the
+ * measured HBM reference for the roofline figures. No counterpart in the reference. */
+int rbfx_stream_copy(rbfx_ctx* ctx, void* d_dst, const void* d_src, uint64_t bytes);
+
 /* Stream-ordered timer: records a HIP event pair on the ctx stream around the work queued
  * between start and stop; stop waits for completion and returns elapsed milliseconds. */
 int rbfx_timer_start(rbfx_ctx* ctx);
@@ -96,6 +104,10 @@ int rbfx_knn_kdtree(int32_t n_nodes, const double* xyz, int32_t k, int32_t* out_
  * uniform +-jitter per coordinate (SplitMix64 counter stream), not re-projected.
  * Stands in for GridReader::readNodeList's input files (src/grids/grid_reader.cpp:36-100). */
 int rbfx_nodes_fibonacci(int32_t n_nodes, double jitter, uint64_t seed, double* xyz);
+/* Icosahedral node set of the given subdivision level, exactly on the unit sphere:
+ * N = 10*4^level + 2 (level 7 = 163842, the quasi-uniform cosine-bell grid size).
+ * With xyz == NULL only *out_n is set. */
+int rbfx_nodes_icosahedral(int32_t level, double* xyz, int32_t* out_n);
 
 /* ---- batched weight computation ------------------------------------------------------ */
 /* RBFFD::computeAllWeightsForAllStencils -> computeAllWeightsForStencil_Direct
@@ -179,13 +191,14 @@ int rbfx_rk4_step(rbfx_rk4* rk, double dt, int32_t n_steps);
 int rbfx_rk4_stage(rbfx_rk4* rk, int32_t stage, double dt);
 
 /* ---- launch-shape tuning (fused kernels; kind 0 = SpMM, 1 = RK4 stage) ----------------- */
-/* Fixes fields-per-thread (1,2,4,8) and block size (64..256, multiple of 64) for this operator
- * set's row count, width, operator count, field count and precisions. rbfx_rk4_step picks
- * the shape up when it (re)captures its graph. */
+/* Fixes fields-per-thread (1,2,4,8), block size (64..256, multiple of 64) and gather batch
+ * (4 or 8 stencil slots per memory round trip) for this operator set's row count, width,
+ * operator count, field count and precisions. rbfx_rk4_step picks the shape up when it
+ * (re)captures its graph. No counterpart in the reference (its launch shape is fixed). */
 int rbfx_tune_set(rbfx_ell* ell, int32_t kind, int32_t n_fields, int32_t field_type, int32_t accum, int32_t fpt,
-                  int32_t block);
+                  int32_t block, int32_t batch);
 int rbfx_tune_get(rbfx_ell* ell, int32_t kind, int32_t n_fields, int32_t field_type, int32_t accum, int32_t* fpt,
-                  int32_t* block);
+                  int32_t* block, int32_t* batch);
 int rbfx_tune_clear(void);
 
 /* ---- cosine bell pointwise terms (src/pdes/hyperbolic/cosine_bell.cpp:23-70) ---------- */

@@ -123,11 +123,23 @@ inline Sph cart2sph(double x, double y, double z) {
 struct Mono {
     int a, b, c;
 };
-inline int mono_list(int deg, Mono* out) {
+// poly_deg carries the degree in its low 4 bits (-1 = none) and, in bit 4 (value 16), the
+// sphere-reduced basis: on the unit sphere |d|^2 + 2 x0.d = 0, so dz^2 is dropped at degree 2
+// and dz^2, x dz^2, y dz^2, dz^3 at degree 3 (9 and 16 functions, the dimensions of the
+// spherical harmonics of degree <= 2 and <= 3).
+const int kPolySphere = 16;
+inline int mono_list(int poly_deg, Mono* out) {
+    if (poly_deg < 0) return 0;
+    int deg = poly_deg & 15;
+    bool sph = (poly_deg & kPolySphere) != 0;
     int cnt = 0;
     for (int d = 0; d <= deg; d++)
         for (int a = d; a >= 0; a--)
-            for (int b = d - a; b >= 0; b--) out[cnt++] = {a, b, d - a - b};
+            for (int b = d - a; b >= 0; b--) {
+                int c = d - a - b;
+                if (sph && c >= 2) continue;  // (0,0,2) (1,0,2) (0,1,2) (0,0,3)
+                out[cnt++] = {a, b, c};
+            }
     return cnt;
 }
 inline double mono_eval(const Mono& m, double dx, double dy, double dz) {
@@ -251,17 +263,19 @@ double orc_hv_pk(double eps, int dim, int k, double r2) { return Gauss(eps, dim)
 
 int orc_num_monomials(int poly_deg) {
     if (poly_deg < 0) return 0;
-    if (poly_deg > 3) return -1;
-    return (poly_deg + 1) * (poly_deg + 2) * (poly_deg + 3) / 6;
+    if ((poly_deg & 15) > 3 || (poly_deg & ~31)) return -1;
+    Mono tmp[20];
+    return mono_list(poly_deg, tmp);
 }
 
 // Degree actually used for a stencil of n nodes: the largest d <= poly_deg whose monomial
 // count does not exceed n (more monomials than nodes make [A P; P' 0] singular, e.g. degree 3
 // needs 20 nodes, so an n = 17 stencil falls back to degree 2). This build's rule.
 int orc_effective_degree(int poly_deg, int n) {
-    int d = poly_deg;
-    while (d > 0 && orc_num_monomials(d) > n) d--;
-    return d;
+    if (poly_deg < 0) return poly_deg;
+    int flag = poly_deg & kPolySphere, d = poly_deg & 15;
+    while (d > 0 && orc_num_monomials(d | flag) > n) d--;
+    return d | flag;
 }
 
 // getStencilLHS + distanceMatrix (rbffd.cpp:129-191, 888-969) and getStencilMultiRHS
@@ -285,7 +299,7 @@ int orc_stencil_system(const double* xyz, const int32_t* st, int32_t n, double e
     }
     const double* x0 = xyz + 3 * (size_t)st[0];
     Mono mono[20];
-    mono_list(poly_deg < 0 ? -1 : poly_deg, mono);
+    mono_list(poly_deg, mono);
     // P block: constant column of ones for the reference's np = 1 (rbffd.cpp:150-156)
     for (int p = 0; p < np; p++) {
         for (int i = 0; i < n; i++) {

@@ -15,8 +15,9 @@ from .lib import (  # noqa: F401
     dertype_bits,
     X, Y, Z, LAPL, R, HV, LAMBDA, THETA, LSFC, XSFC, YSFC, ZSFC, INTERP,
     F64, F32,
-    SOLVE_F64_LU, SOLVE_F32_LU_F64_IR,
+    SOLVE_F64_LU, SOLVE_F32_LU_F64_IR, POLY_SPHERE,
     knn_kdtree,
     nodes_fibonacci,
+    nodes_icosahedral,
     csr_from_fixed,
 )

@@ -42,7 +42,7 @@ __global__ void ell_validate_kernel(const int32_t* __restrict__ idx, size_t tota
     if (v < 0 || v >= n_cols) atomicAdd(bad, 1);
 }
 
-template <typename WT, typename UT, typename AT, int NOPS, int FPT>
+template <typename WT, typename UT, typename AT, int NOPS, int FPT, int UB>
 __global__ void __launch_bounds__(256) ell_spmm_kernel(const int32_t* __restrict__ idx_t, const WT* __restrict__ w_t,
                                                        const UT* __restrict__ u, UT* __restrict__ out, int n_rows,
                                                        int n_pad, int width, int F, int n_tiles, int nb) {
@@ -57,7 +57,7 @@ __global__ void __launch_bounds__(256) ell_spmm_kernel(const int32_t* __restrict
     if (row >= n_rows) return;
     int fofs = (t - rl * G) * FPT;
     AT acc[NOPS][FPT];
-    ell_gather_dot<WT, UT, AT, NOPS, FPT>(idx_t, w_t, u, row, n_pad, width, F, fofs, acc);
+    ell_gather_dot<WT, UT, AT, NOPS, FPT, UB>(idx_t, w_t, u, row, n_pad, width, F, fofs, acc);
 #pragma unroll
     for (int o = 0; o < NOPS; o++) {
         UT r[FPT];
@@ -89,7 +89,8 @@ __global__ void __launch_bounds__(256) ell_spmv_strided_kernel(const int32_t* __
 inline int grid_blocks(int n_tiles) { return (n_tiles + 7) / 8 * 8; }
 
 template <typename WT, typename UT, typename AT, int NOPS>
-int launch_spmm_fpt(rbfx_ell* e, int fpt, int block, int F, const void* u, void* out) {
+int launch_spmm_fpt(rbfx_ell* e, const EllTune& t, int F, const void* u, void* out) {
+    const int fpt = t.fpt, block = t.block;
     int G = F / fpt;
     int R = block / G;
     if (R <= 0) return rbfx_fail(e->ctx, RBFX_E_INVALID, "spmm: %d fields with %d per thread exceed block %d", F, fpt, block);
@@ -100,24 +101,26 @@ int launch_spmm_fpt(rbfx_ell* e, int fpt, int block, int F, const void* u, void*
     const UT* uu = (const UT*)u;
     UT* oo = (UT*)out;
     hipStream_t s = e->ctx->stream;
-    switch (fpt) {
-        case 1: ell_spmm_kernel<WT, UT, AT, NOPS, 1><<<nb, block, 0, s>>>(it, wt, uu, oo, e->n_rows, e->n_pad, e->width, F, n_tiles, nb); break;
-        case 2: ell_spmm_kernel<WT, UT, AT, NOPS, 2><<<nb, block, 0, s>>>(it, wt, uu, oo, e->n_rows, e->n_pad, e->width, F, n_tiles, nb); break;
-        case 4: ell_spmm_kernel<WT, UT, AT, NOPS, 4><<<nb, block, 0, s>>>(it, wt, uu, oo, e->n_rows, e->n_pad, e->width, F, n_tiles, nb); break;
-        case 8: ell_spmm_kernel<WT, UT, AT, NOPS, 8><<<nb, block, 0, s>>>(it, wt, uu, oo, e->n_rows, e->n_pad, e->width, F, n_tiles, nb); break;
-        default: return rbfx_fail(e->ctx, RBFX_E_INVALID, "spmm: unsupported fields-per-thread %d", fpt);
+#define RBFX_SPMM_CASE(FPT_, UB_)                                                                                   \
+    if (fpt == FPT_ && t.ub == UB_) {                                                                               \
+        ell_spmm_kernel<WT, UT, AT, NOPS, FPT_, UB_><<<nb, block, 0, s>>>(it, wt, uu, oo, e->n_rows, e->n_pad,      \
+                                                                           e->width, F, n_tiles, nb);               \
+        RBFX_LAUNCH_CHECK(e->ctx, "ell_spmm_kernel");                                                              \
+        return RBFX_OK;                                                                                             \
     }
-    RBFX_LAUNCH_CHECK(e->ctx, "ell_spmm_kernel");
-    return RBFX_OK;
+    RBFX_SPMM_CASE(1, 4) RBFX_SPMM_CASE(2, 4) RBFX_SPMM_CASE(4, 4) RBFX_SPMM_CASE(8, 4)
+    RBFX_SPMM_CASE(1, 8) RBFX_SPMM_CASE(2, 8) RBFX_SPMM_CASE(4, 8) RBFX_SPMM_CASE(8, 8)
+#undef RBFX_SPMM_CASE
+    return rbfx_fail(e->ctx, RBFX_E_INVALID, "spmm: unsupported shape (fields/thread %d, batch %d)", fpt, t.ub);
 }
 
 template <typename WT, typename UT, typename AT>
-int launch_spmm_ops(rbfx_ell* e, int fpt, int block, int F, const void* u, void* out) {
+int launch_spmm_ops(rbfx_ell* e, const EllTune& t, int F, const void* u, void* out) {
     switch (e->n_ops) {
-        case 1: return launch_spmm_fpt<WT, UT, AT, 1>(e, fpt, block, F, u, out);
-        case 2: return launch_spmm_fpt<WT, UT, AT, 2>(e, fpt, block, F, u, out);
-        case 3: return launch_spmm_fpt<WT, UT, AT, 3>(e, fpt, block, F, u, out);
-        case 4: return launch_spmm_fpt<WT, UT, AT, 4>(e, fpt, block, F, u, out);
+        case 1: return launch_spmm_fpt<WT, UT, AT, 1>(e, t, F, u, out);
+        case 2: return launch_spmm_fpt<WT, UT, AT, 2>(e, t, F, u, out);
+        case 3: return launch_spmm_fpt<WT, UT, AT, 3>(e, t, F, u, out);
+        case 4: return launch_spmm_fpt<WT, UT, AT, 4>(e, t, F, u, out);
         default: return rbfx_fail(e->ctx, RBFX_E_TOO_LARGE, "spmm: fused application supports 1..4 operators, set has %d", e->n_ops);
     }
 }
@@ -228,10 +231,10 @@ int rbfx_spmm(rbfx_ell* e, int32_t n_fields, const void* d_u, void* d_out, int32
     EllTune t;
     rc = rbfx_ell_pick(e, n_fields, field_type, accum, 0, &t);
     if (rc) return rc;
-    if (e->storage == RBFX_F64) return launch_spmm_ops<double, double, double>(e, t.fpt, t.block, n_fields, d_u, d_out);
-    if (field_type == RBFX_F32 && accum == RBFX_F64) return launch_spmm_ops<float, float, double>(e, t.fpt, t.block, n_fields, d_u, d_out);
-    if (field_type == RBFX_F32) return launch_spmm_ops<float, float, float>(e, t.fpt, t.block, n_fields, d_u, d_out);
-    return launch_spmm_ops<float, double, double>(e, t.fpt, t.block, n_fields, d_u, d_out);
+    if (e->storage == RBFX_F64) return launch_spmm_ops<double, double, double>(e, t, n_fields, d_u, d_out);
+    if (field_type == RBFX_F32 && accum == RBFX_F64) return launch_spmm_ops<float, float, double>(e, t, n_fields, d_u, d_out);
+    if (field_type == RBFX_F32) return launch_spmm_ops<float, float, float>(e, t, n_fields, d_u, d_out);
+    return launch_spmm_ops<float, double, double>(e, t, n_fields, d_u, d_out);
 }
 
 int rbfx_spmv(rbfx_ell* e, int32_t op, const void* d_u, void* d_out, int32_t field_type, int32_t accum) {
@@ -243,10 +246,10 @@ int rbfx_spmv(rbfx_ell* e, int32_t op, const void* d_u, void* d_out, int32_t fie
         EllTune t;
         rc = rbfx_ell_pick(e, 1, field_type, accum, 0, &t);
         if (rc) return rc;
-        if (e->storage == RBFX_F64) return launch_spmm_fpt<double, double, double, 1>(e, 1, t.block, 1, d_u, d_out);
-        if (field_type == RBFX_F32 && accum == RBFX_F64) return launch_spmm_fpt<float, float, double, 1>(e, 1, t.block, 1, d_u, d_out);
-        if (field_type == RBFX_F32) return launch_spmm_fpt<float, float, float, 1>(e, 1, t.block, 1, d_u, d_out);
-        return launch_spmm_fpt<float, double, double, 1>(e, 1, t.block, 1, d_u, d_out);
+        if (e->storage == RBFX_F64) return launch_spmm_fpt<double, double, double, 1>(e, t, 1, d_u, d_out);
+        if (field_type == RBFX_F32 && accum == RBFX_F64) return launch_spmm_fpt<float, float, double, 1>(e, t, 1, d_u, d_out);
+        if (field_type == RBFX_F32) return launch_spmm_fpt<float, float, float, 1>(e, t, 1, d_u, d_out);
+        return launch_spmm_fpt<float, double, double, 1>(e, t, 1, d_u, d_out);
     }
     const int block = 256;
     int n_tiles = (e->n_rows + block - 1) / block;

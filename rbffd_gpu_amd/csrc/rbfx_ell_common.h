@@ -90,7 +90,13 @@ template <>
 __device__ __forceinline__ float fma_acc<float>(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 
 // acc[o][f] = sum_j w_o[row][j] * u[idx[row][j]*F + fofs + f], j ascending.
-template <typename WT, typename UT, typename AT, int NOPS, int FPT>
+//
+// Slots are processed in batches of UB: the UB neighbour records of a batch are gathered
+// back to back, and the indices and weights of the next batch are requested before the
+// current batch is consumed, so one memory round trip covers UB slots instead of the
+// index-load -> gather dependency costing two round trips per slot. The summation order per
+// accumulator stays j = 0, 1, ..., width-1 whatever UB is.
+template <typename WT, typename UT, typename AT, int NOPS, int FPT, int UB>
 __device__ __forceinline__ void ell_gather_dot(const int32_t* __restrict__ idx_t, const WT* __restrict__ w_t,
                                                const UT* __restrict__ u, int row, int n_pad, int width, int F,
                                                int fofs, AT (&acc)[NOPS][FPT]) {
@@ -101,41 +107,56 @@ __device__ __forceinline__ void ell_gather_dot(const int32_t* __restrict__ idx_t
     const int32_t* ip = idx_t + row;
     const WT* wp = w_t + (size_t)row * NOPS;
     const size_t wstride = (size_t)n_pad * NOPS;
-    int j = 0;
-    // two slots per iteration keeps two independent gathers in flight per thread
-    for (; j + 1 < width; j += 2) {
-        int n0 = __builtin_nontemporal_load(ip + (size_t)j * n_pad);
-        int n1 = __builtin_nontemporal_load(ip + (size_t)(j + 1) * n_pad);
-        WT w0[NOPS], w1[NOPS];
-        load_w<WT, NOPS>(wp + (size_t)j * wstride, w0);
-        load_w<WT, NOPS>(wp + (size_t)(j + 1) * wstride, w1);
-        UT v0[FPT], v1[FPT];
-        load_pack<UT, FPT>(u + (size_t)n0 * F + fofs, v0);
-        load_pack<UT, FPT>(u + (size_t)n1 * F + fofs, v1);
+    const UT* ub = u + fofs;
+    int nb[UB];
+    WT w[UB][NOPS];
 #pragma unroll
-        for (int o = 0; o < NOPS; o++) {
-            AT a0 = (AT)w0[o];
+    for (int q = 0; q < UB; q++) {
+        if (q < width) {
+            nb[q] = __builtin_nontemporal_load(ip + (size_t)q * n_pad);
+            load_w<WT, NOPS>(wp + (size_t)q * wstride, w[q]);
+        } else {
+            nb[q] = 0;
 #pragma unroll
-            for (int f = 0; f < FPT; f++) acc[o][f] = fma_acc<AT>(a0, (AT)v0[f], acc[o][f]);
-        }
-#pragma unroll
-        for (int o = 0; o < NOPS; o++) {
-            AT a1 = (AT)w1[o];
-#pragma unroll
-            for (int f = 0; f < FPT; f++) acc[o][f] = fma_acc<AT>(a1, (AT)v1[f], acc[o][f]);
+            for (int o = 0; o < NOPS; o++) w[q][o] = (WT)0;
         }
     }
-    if (j < width) {
-        int n0 = ip[(size_t)j * n_pad];
-        WT w0[NOPS];
-        load_w<WT, NOPS>(wp + (size_t)j * wstride, w0);
-        UT v0[FPT];
-        load_pack<UT, FPT>(u + (size_t)n0 * F + fofs, v0);
+    for (int j0 = 0; j0 < width; j0 += UB) {
+        UT v[UB][FPT];
 #pragma unroll
-        for (int o = 0; o < NOPS; o++) {
-            AT a0 = (AT)w0[o];
+        for (int q = 0; q < UB; q++) {
+            if (j0 + q < width) load_pack<UT, FPT>(ub + (size_t)((unsigned)nb[q] * (unsigned)F), v[q]);
+        }
+        int nb2[UB];
+        WT w2[UB][NOPS];
 #pragma unroll
-            for (int f = 0; f < FPT; f++) acc[o][f] = fma_acc<AT>(a0, (AT)v0[f], acc[o][f]);
+        for (int q = 0; q < UB; q++) {
+            int j = j0 + UB + q;
+            if (j < width) {
+                nb2[q] = __builtin_nontemporal_load(ip + (size_t)j * n_pad);
+                load_w<WT, NOPS>(wp + (size_t)j * wstride, w2[q]);
+            } else {
+                nb2[q] = 0;
+#pragma unroll
+                for (int o = 0; o < NOPS; o++) w2[q][o] = (WT)0;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < UB; q++) {
+            if (j0 + q < width) {
+#pragma unroll
+                for (int o = 0; o < NOPS; o++) {
+                    AT a = (AT)w[q][o];
+#pragma unroll
+                    for (int f = 0; f < FPT; f++) acc[o][f] = fma_acc<AT>(a, (AT)v[q][f], acc[o][f]);
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < UB; q++) {
+            nb[q] = nb2[q];
+#pragma unroll
+            for (int o = 0; o < NOPS; o++) w[q][o] = w2[q][o];
         }
     }
 }
@@ -152,6 +173,7 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
 struct EllTune {
     int fpt = 1;
     int block = 256;
+    int ub = 4;  // slots per gather batch (4 or 8)
 };
 
 int rbfx_ell_pick(rbfx_ell* ell, int n_fields, int field_type, int accum, int kind, EllTune* t);

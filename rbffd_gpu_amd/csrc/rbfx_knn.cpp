@@ -23,6 +23,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <unordered_map>
 #include <vector>
 #ifdef _OPENMP
 #include <omp.h>
@@ -283,6 +284,55 @@ int rbfx_knn_kdtree(int32_t n_nodes, const double* xyz, int32_t k, int32_t* out_
         for (int32_t i = 0; i < n_nodes; i++) tree.knn(xyz + 3 * (size_t)i, k, pq, out_idx + (size_t)i * k);
     }
     (void)n_threads;
+    return RBFX_OK;
+}
+
+// Icosahedral node set: the 12 vertices of an icosahedron, each face split into four
+// `level` times with every new edge midpoint projected onto the unit sphere;
+// N = 10 * 4^level + 2 (level 7: 163842). Vertex order: the 12 corners, then midpoints in
+// creation order. out_n receives N; xyz may be NULL to query the count.
+int rbfx_nodes_icosahedral(int32_t level, double* xyz, int32_t* out_n) {
+    if (level < 0 || level > 10 || !out_n) return RBFX_E_INVALID;
+    int64_t n_total = 10 * (int64_t(1) << (2 * level)) + 2;
+    *out_n = (int32_t)n_total;
+    if (!xyz) return RBFX_OK;
+    const double t = (1.0 + std::sqrt(5.0)) / 2.0;
+    std::vector<double> v = {-1, t, 0, 1, t, 0, -1, -t, 0, 1, -t, 0, 0, -1, t, 0, 1, t,
+                             0, -1, -t, 0, 1, -t, t, 0, -1, t, 0, 1, -t, 0, -1, -t, 0, 1};
+    for (int i = 0; i < 12; i++) {
+        double r = std::sqrt(v[3 * i] * v[3 * i] + v[3 * i + 1] * v[3 * i + 1] + v[3 * i + 2] * v[3 * i + 2]);
+        for (int d = 0; d < 3; d++) v[3 * i + d] /= r;
+    }
+    std::vector<int> f = {0, 11, 5, 0, 5, 1, 0, 1, 7, 0, 7, 10, 0, 10, 11, 1, 5, 9, 5, 11, 4, 11, 10, 2, 10, 7, 6, 7, 1, 8,
+                          3, 9, 4, 3, 4, 2, 3, 2, 6, 3, 6, 8, 3, 8, 9, 4, 9, 5, 2, 4, 11, 6, 2, 10, 8, 6, 7, 9, 8, 1};
+    v.reserve(3 * (size_t)n_total);
+    for (int l = 0; l < level; l++) {
+        std::vector<int> nf;
+        nf.reserve(f.size() * 4);
+        std::unordered_map<uint64_t, int> mid;
+        mid.reserve(f.size());
+        auto midpoint = [&](int a, int b) {
+            uint64_t key = ((uint64_t)std::min(a, b) << 32) | (uint64_t)std::max(a, b);
+            auto it = mid.find(key);
+            if (it != mid.end()) return it->second;
+            double p[3];
+            for (int d = 0; d < 3; d++) p[d] = 0.5 * (v[3 * (size_t)a + d] + v[3 * (size_t)b + d]);
+            double r = std::sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+            int id = (int)(v.size() / 3);
+            for (int d = 0; d < 3; d++) v.push_back(p[d] / r);
+            mid.emplace(key, id);
+            return id;
+        };
+        for (size_t q = 0; q < f.size(); q += 3) {
+            int a = f[q], b = f[q + 1], c = f[q + 2];
+            int ab = midpoint(a, b), bc = midpoint(b, c), ca = midpoint(c, a);
+            int tri[12] = {a, ab, ca, b, bc, ab, c, ca, bc, ab, bc, ca};
+            nf.insert(nf.end(), tri, tri + 12);
+        }
+        f.swap(nf);
+    }
+    if ((int64_t)(v.size() / 3) != n_total) return RBFX_E_INVALID;
+    std::memcpy(xyz, v.data(), v.size() * sizeof(double));
     return RBFX_OK;
 }
 

@@ -52,7 +52,7 @@ struct StageArgs {
     int n_tiles, nb;
 };
 
-template <typename WT, typename UT, typename AT, int NOPS, int FPT>
+template <typename WT, typename UT, typename AT, int NOPS, int FPT, int UB>
 __global__ void __launch_bounds__(256) rk4_stage_kernel(StageArgs a) {
     const int G = a.F / FPT;
     const int R = blockDim.x / G;
@@ -65,7 +65,7 @@ __global__ void __launch_bounds__(256) rk4_stage_kernel(StageArgs a) {
     if (row >= a.n_rows) return;
     int fofs = (t - rl * G) * FPT;
     AT d[NOPS][FPT];
-    ell_gather_dot<WT, UT, AT, NOPS, FPT>(a.idx_t, (const WT*)a.w_t, (const UT*)a.xin, row, a.n_pad, a.width, a.F, fofs, d);
+    ell_gather_dot<WT, UT, AT, NOPS, FPT, UB>(a.idx_t, (const WT*)a.w_t, (const UT*)a.xin, row, a.n_pad, a.width, a.F, fofs, d);
     double c[NOPS];
 #pragma unroll
     for (int o = 0; o < NOPS; o++) c[o] = a.coef[o] ? a.coef[o][row] : 1.0;
@@ -122,15 +122,17 @@ template <typename WT, typename UT, typename AT, int NOPS>
 int launch_stage_fpt(rbfx_rk4* rk, StageArgs& a, int block) {
     hipStream_t s = rk->ctx->stream;
     int nb = a.nb;
-    switch (rk->tune.fpt) {
-        case 1: rk4_stage_kernel<WT, UT, AT, NOPS, 1><<<nb, block, 0, s>>>(a); break;
-        case 2: rk4_stage_kernel<WT, UT, AT, NOPS, 2><<<nb, block, 0, s>>>(a); break;
-        case 4: rk4_stage_kernel<WT, UT, AT, NOPS, 4><<<nb, block, 0, s>>>(a); break;
-        case 8: rk4_stage_kernel<WT, UT, AT, NOPS, 8><<<nb, block, 0, s>>>(a); break;
-        default: return rbfx_fail(rk->ctx, RBFX_E_INVALID, "rk4: unsupported fields-per-thread %d", rk->tune.fpt);
+    const int fpt = rk->tune.fpt, ub = rk->tune.ub;
+#define RBFX_STAGE_CASE(FPT_, UB_)                                           \
+    if (fpt == FPT_ && ub == UB_) {                                          \
+        rk4_stage_kernel<WT, UT, AT, NOPS, FPT_, UB_><<<nb, block, 0, s>>>(a); \
+        RBFX_LAUNCH_CHECK(rk->ctx, "rk4_stage_kernel");                     \
+        return RBFX_OK;                                                      \
     }
-    RBFX_LAUNCH_CHECK(rk->ctx, "rk4_stage_kernel");
-    return RBFX_OK;
+    RBFX_STAGE_CASE(1, 4) RBFX_STAGE_CASE(2, 4) RBFX_STAGE_CASE(4, 4) RBFX_STAGE_CASE(8, 4)
+    RBFX_STAGE_CASE(1, 8) RBFX_STAGE_CASE(2, 8) RBFX_STAGE_CASE(4, 8) RBFX_STAGE_CASE(8, 8)
+#undef RBFX_STAGE_CASE
+    return rbfx_fail(rk->ctx, RBFX_E_INVALID, "rk4: unsupported shape (fields/thread %d, batch %d)", fpt, ub);
 }
 
 template <typename WT, typename UT, typename AT>
@@ -278,7 +280,7 @@ int rbfx_rk4_step(rbfx_rk4* rk, double dt, int32_t n_steps) {
     EllTune t;
     int prc = rbfx_ell_pick(rk->ell, rk->F, rk->field_storage, rk->accum, 1, &t);
     if (prc) return prc;
-    bool shape_changed = (t.fpt != rk->tune.fpt || t.block != rk->tune.block);
+    bool shape_changed = (t.fpt != rk->tune.fpt || t.block != rk->tune.block || t.ub != rk->tune.ub);
     rk->tune = t;
     if (rk->use_graph) {
         if (!rk->exec || rk->graph_dt != dt || shape_changed) {

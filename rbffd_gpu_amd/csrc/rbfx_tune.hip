@@ -59,21 +59,26 @@ int rbfx_ell_pick(rbfx_ell* e, int F, int ft, int acc, int kind, EllTune* t) {
 extern "C" {
 
 // Sets the launch shape used for (ell, n_fields, types, kind); kind 0 = SpMM, 1 = RK4 stage.
-int rbfx_tune_set(rbfx_ell* e, int32_t kind, int32_t n_fields, int32_t field_type, int32_t accum, int32_t fpt, int32_t block) {
+int rbfx_tune_set(rbfx_ell* e, int32_t kind, int32_t n_fields, int32_t field_type, int32_t accum, int32_t fpt, int32_t block,
+                  int32_t batch) {
     if (!e) return RBFX_E_INVALID;
-    if (!valid(n_fields, fpt, block)) return rbfx_fail(e->ctx, RBFX_E_INVALID, "tune_set: invalid shape fpt %d block %d for %d fields", fpt, block, n_fields);
+    if (!valid(n_fields, fpt, block) || (batch != 4 && batch != 8))
+        return rbfx_fail(e->ctx, RBFX_E_INVALID, "tune_set: invalid shape fpt %d block %d batch %d for %d fields", fpt, block,
+                         batch, n_fields);
     std::lock_guard<std::mutex> lk(g_mu);
-    g_table[make_key(e, n_fields, field_type, accum, kind)] = EllTune{fpt, block};
+    g_table[make_key(e, n_fields, field_type, accum, kind)] = EllTune{fpt, block, batch};
     return RBFX_OK;
 }
 
-int rbfx_tune_get(rbfx_ell* e, int32_t kind, int32_t n_fields, int32_t field_type, int32_t accum, int32_t* fpt, int32_t* block) {
-    if (!e || !fpt || !block) return RBFX_E_INVALID;
+int rbfx_tune_get(rbfx_ell* e, int32_t kind, int32_t n_fields, int32_t field_type, int32_t accum, int32_t* fpt, int32_t* block,
+                  int32_t* batch) {
+    if (!e || !fpt || !block || !batch) return RBFX_E_INVALID;
     EllTune t;
     int rc = rbfx_ell_pick(e, n_fields, field_type, accum, kind, &t);
     if (rc) return rc;
     *fpt = t.fpt;
     *block = t.block;
+    *batch = t.ub;
     return RBFX_OK;
 }
 

@@ -37,14 +37,26 @@ __constant__ Mono3 c_mono[20] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {2,
                                  {0, 2, 0}, {0, 1, 1}, {0, 0, 2}, {3, 0, 0}, {2, 1, 0}, {2, 0, 1}, {1, 2, 0},
                                  {1, 1, 1}, {1, 0, 2}, {0, 3, 0}, {0, 2, 1}, {0, 1, 2}, {0, 0, 3}};
 
-int num_monomials(int deg) { return deg < 0 ? 0 : (deg + 1) * (deg + 2) * (deg + 3) / 6; }
+// Sphere-reduced basis (poly_deg | RBFX_POLY_SPHERE): on the unit sphere |d|^2 + 2 x0.d = 0,
+// so the monomials with a power of z of 2 or more are dropped: 9 functions at degree 2, 16 at
+// degree 3 (the dimensions of the spherical harmonics up to that degree).
+__constant__ Mono3 c_mono_sph[16] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {2, 0, 0}, {1, 1, 0},
+                                     {1, 0, 1}, {0, 2, 0}, {0, 1, 1}, {3, 0, 0}, {2, 1, 0}, {2, 0, 1},
+                                     {1, 2, 0}, {1, 1, 1}, {0, 3, 0}, {0, 2, 1}};
+
+int num_monomials(int poly_deg) {
+    if (poly_deg < 0) return 0;
+    int d = poly_deg & 15;
+    return (poly_deg & RBFX_POLY_SPHERE) ? (d + 1) * (d + 1) : (d + 1) * (d + 2) * (d + 3) / 6;
+}
 
 // Degree used for a stencil of n nodes: the largest d <= poly_deg with at most n monomials
 // (more monomials than nodes make [A P; P' 0] singular; degree 3 needs 20 nodes).
 int effective_degree(int poly_deg, int n) {
-    int d = poly_deg;
-    while (d > 0 && num_monomials(d) > n) d--;
-    return d;
+    if (poly_deg < 0) return poly_deg;
+    int flag = poly_deg & RBFX_POLY_SPHERE, d = poly_deg & 15;
+    while (d > 0 && num_monomials(d | flag) > n) d--;
+    return d | flag;
 }
 
 struct WParams {
@@ -66,6 +78,7 @@ struct WParams {
     int* fail_flags;       // per stencil: 1 = singular / not converged (FP32 mode)
     int max_iter;
     int fp32;
+    int sph;               // sphere-reduced polynomial basis
 };
 
 __device__ __forceinline__ double sq3(double x, double y, double z) { return x * x + y * y + z * z; }
@@ -130,8 +143,8 @@ __device__ double rbf_rhs(uint32_t type, double eps, double e2, int dim, int hv_
     }
 }
 
-__device__ double mono_eval(int p, double dx, double dy, double dz) {
-    Mono3 mm = c_mono[p];
+__device__ double mono_eval(int p, int sph, double dx, double dy, double dz) {
+    Mono3 mm = sph ? c_mono_sph[p] : c_mono[p];
     double v = 1.0;
     for (int i = 0; i < mm.a; i++) v *= dx;
     for (int i = 0; i < mm.b; i++) v *= dy;
@@ -140,8 +153,8 @@ __device__ double mono_eval(int p, double dx, double dy, double dz) {
 }
 
 // Operator applied to monomial p at the centre (d = 0): only degree 0..2 terms contribute.
-__device__ double mono_rhs(uint32_t type, int p, const double* x0, int dim, int hv_k, double lam0, double th0) {
-    Mono3 mm = c_mono[p];
+__device__ double mono_rhs(uint32_t type, int p, int sph, const double* x0, int dim, int hv_k, double lam0, double th0) {
+    Mono3 mm = sph ? c_mono_sph[p] : c_mono[p];
     int deg = mm.a + mm.b + mm.c;
     double p0 = (deg == 0) ? 1.0 : 0.0;
     double g[3] = {0, 0, 0};
@@ -206,13 +219,13 @@ __host__ __device__ inline size_t lds_bytes(int m, int n, int T, bool ir) {
 }
 
 // Entry (r, c) of the FP64 saddle-point matrix, from the staged coordinates.
-__device__ __forceinline__ double m_entry(int r, int c, int n, double e2, const double* X) {
+__device__ __forceinline__ double m_entry(int r, int c, int n, double e2, const double* X, int sph) {
     if (r < n && c < n)
         return g_eval(e2, X[3 * c] - X[3 * r], X[3 * c + 1] - X[3 * r + 1], X[3 * c + 2] - X[3 * r + 2]);
     if (r >= n && c >= n) return 0.0;
     int i = (r < n) ? r : c;
     int p = (r < n) ? c - n : r - n;
-    return mono_eval(p, X[3 * i] - X[0], X[3 * i + 1] - X[1], X[3 * i + 2] - X[2]);
+    return mono_eval(p, sph, X[3 * i] - X[0], X[3 * i + 1] - X[1], X[3 * i + 2] - X[2]);
 }
 
 struct Sys {
@@ -236,7 +249,7 @@ __device__ __forceinline__ double rhs_entry(const Sys& S, int r, int t) {
     if (r < S.n)
         return rbf_rhs(P.types[t], S.eps, S.e2, P.dim, P.hv_k, S.X, S.X + 3 * r, lam0, th0, S.sph ? S.SPH[2 * r] : 0.,
                        S.sph ? S.SPH[2 * r + 1] : 0.);
-    return mono_rhs(P.types[t], r - S.n, S.X, P.dim, P.hv_k, lam0, th0);
+    return mono_rhs(P.types[t], r - S.n, P.sph, S.X, P.dim, P.hv_k, lam0, th0);
 }
 
 // argmax over the block (ties -> smaller row); result in *out_p, visible after the caller's barrier.
@@ -354,7 +367,7 @@ __device__ void residual(Sys& S, TA* Aug, const double* XS, double* out_be) {
             bmax[t] = fabs(b);
         }
         for (int c = 0; c < m; c++) {
-            double a = m_entry(tid, c, S.n, S.e2, S.X);
+            double a = m_entry(tid, c, S.n, S.e2, S.X, S.P->sph);
             rowsum += fabs(a);
             for (int t = 0; t < T; t++) acc[t] -= a * XS[c * T + t];
         }
@@ -470,7 +483,7 @@ __device__ double growth(const Sys& S, const TA* Aug) {
     for (int q = threadIdx.x; q < S.m * S.m; q += blockDim.x) {
         int r = q / S.m, c = q - r * S.m;
         if (c >= S.pos[r]) umax = fmax(umax, fabs((double)Aug[(size_t)r * S.ld + c]));
-        amax = fmax(amax, fabs(m_entry(r, c, S.n, S.e2, S.X)));
+        amax = fmax(amax, fabs(m_entry(r, c, S.n, S.e2, S.X, S.P->sph)));
     }
     for (int off = 32; off > 0; off >>= 1) {
         umax = fmax(umax, __shfl_xor(umax, off, 64));
@@ -502,7 +515,7 @@ __global__ void __launch_bounds__(WBLOCK) weights_f64_kernel(WParams P) {
     __syncthreads();
     for (int q = tid; q < m * m; q += nthr) {
         int r = q / m, c = q - r * m;
-        Aug[(size_t)r * ld + c] = m_entry(r, c, S.n, S.e2, S.X);
+        Aug[(size_t)r * ld + c] = m_entry(r, c, S.n, S.e2, S.X, S.P->sph);
     }
     for (int q = tid; q < m * T; q += nthr) {
         int r = q / T, t = q - r * T;
@@ -559,7 +572,7 @@ __global__ void __launch_bounds__(WBLOCK) weights_f32ir_kernel(WParams P) {
     __syncthreads();
     for (int q = tid; q < m * m; q += nthr) {
         int r = q / m, c = q - r * m;
-        Aug[(size_t)r * ld + c] = (float)m_entry(r, c, S.n, S.e2, S.X);
+        Aug[(size_t)r * ld + c] = (float)m_entry(r, c, S.n, S.e2, S.X, S.P->sph);
     }
     for (int q = tid; q < m * T; q += nthr) {
         int r = q / T, t = q - r * T;
@@ -650,7 +663,8 @@ extern "C" int rbfx_weights(rbfx_ctx* ctx, int32_t n_stencils, const int32_t* d_
     if (!dertypes || (dertypes & ~supported))
         return rbfx_fail(ctx, RBFX_E_INVALID, "weights: unsupported derivative mask 0x%x", dertypes);
     if (dim < 1 || dim > 3) return rbfx_fail(ctx, RBFX_E_INVALID, "weights: dim %d not in 1..3", dim);
-    if (poly_deg < -1 || poly_deg > 3) return rbfx_fail(ctx, RBFX_E_INVALID, "weights: poly_deg %d not in -1..3", poly_deg);
+    if (poly_deg < -1 || (poly_deg >= 0 && ((poly_deg & 15) > 3 || (poly_deg & ~(15 | RBFX_POLY_SPHERE)))))
+        return rbfx_fail(ctx, RBFX_E_INVALID, "weights: poly_deg %d not in -1..3 (| RBFX_POLY_SPHERE)", poly_deg);
     if (hv_k < 0 || hv_k > 16) return rbfx_fail(ctx, RBFX_E_INVALID, "weights: hv_k %d", hv_k);
     if (mode != RBFX_SOLVE_F64_LU && mode != RBFX_SOLVE_F32_LU_F64_IR) return rbfx_fail(ctx, RBFX_E_INVALID, "weights: mode %d", mode);
     RBFX_HIP_CHECK(ctx, hipSetDevice(ctx->device));
@@ -663,6 +677,7 @@ extern "C" int rbfx_weights(rbfx_ctx* ctx, int32_t n_stencils, const int32_t* d_
         if (dertypes & (1u << b)) P.types[T++] = 1u << b;
     P.T = T;
     P.np = num_monomials(poly_deg);
+    P.sph = (poly_deg >= 0 && (poly_deg & RBFX_POLY_SPHERE)) ? 1 : 0;
     P.dim = dim;
     P.hv_k = hv_k;
     P.hv_scale = hv_scale;

@@ -1,0 +1,142 @@
+#include "grid.h"
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <iostream>
+#include <sstream>
+
+#include "rbfx.h"
+
+namespace rbfx {
+
+void Grid::checkStencilSize() {
+    unsigned int nb_rbf = (unsigned)node_list.size();
+    if (max_st_size < 1) {
+        std::cout << "[Grid] ERROR! Stencil size must be >= 1" << std::endl;
+        std::exit(EXIT_FAILURE);
+    }
+    if (max_st_size > nb_rbf) {
+        std::cout << "[Grid] WARNING! Not enough nodes to reach specified stencil_size (size: " << max_st_size
+                  << ")! Using new stencil size: " << nb_rbf << std::endl;
+        max_st_size = nb_rbf > 1 ? nb_rbf : 1;
+    }
+}
+
+void Grid::generateStencils(unsigned int st_max_size, st_generator_t generator_choice) {
+    max_st_size = st_max_size;
+    generateStencils(generator_choice);
+}
+
+void Grid::generateStencils(st_generator_t generator_choice) {
+    if (generator_choice != ST_KDTREE) {
+        std::cout << "[Grid] ERROR: only the kd-tree stencil generator (ST_KDTREE) is provided" << std::endl;
+        std::exit(EXIT_FAILURE);
+    }
+    checkStencilSize();
+    size_t n = node_list.size();
+    std::vector<double> xyz(3 * n);
+    for (size_t i = 0; i < n; i++)
+        for (int d = 0; d < 3; d++) xyz[3 * i + d] = node_list[i][d];
+    std::vector<int32_t> st(n * (size_t)max_st_size);
+    int rc = rbfx_knn_kdtree((int32_t)n, xyz.data(), (int32_t)max_st_size, st.data(), 0);
+    if (rc != RBFX_OK) {
+        std::cout << "[Grid] ERROR: kd-tree stencil generation failed (" << rc << ")" << std::endl;
+        std::exit(EXIT_FAILURE);
+    }
+    stencil_map.assign(n, StencilType());
+    for (size_t i = 0; i < n; i++) {
+        stencil_map[i].resize(max_st_size);
+        for (unsigned j = 0; j < max_st_size; j++) stencil_map[i][j] = st[i * max_st_size + j];
+        // radius trimming against max_st_radius (DBL_MAX by default, as in the reference)
+        const NodeType& c = node_list[i];
+        for (unsigned j = 0; j < max_st_size; j++) {
+            const NodeType& p = node_list[stencil_map[i][j]];
+            double dx = p[0] - c[0], dy = p[1] - c[1], dz = p[2] - c[2];
+            if (!(std::sqrt(dx * dx + dy * dy + dz * dz) < max_st_radius)) {
+                stencil_map[i].resize(j);
+                break;
+            }
+        }
+    }
+    computeStencilRadii();
+}
+
+void Grid::computeStencilRadii() {
+    size_t nb = stencil_map.size();
+    avg_stencil_radii.assign(nb, 0.);
+    min_stencil_radii.assign(nb, 0.);
+    max_stencil_radii.assign(nb, 0.);
+    auto dist = [&](int a, int b) {
+        NodeType d(node_list[a][0] - node_list[b][0], node_list[a][1] - node_list[b][1], node_list[a][2] - node_list[b][2]);
+        return std::sqrt(d.square());
+    };
+    for (size_t i = 0; i < nb; i++) {
+        StencilType& st = stencil_map[i];
+        if (st.size() < 2) continue;
+        min_stencil_radii[i] = dist(st[1], st[0]);
+        max_stencil_radii[i] = dist(st[st.size() - 1], st[0]);
+        double s = 0.;
+        for (size_t k = 1; k < st.size(); k++) s += dist(st[k], st[0]);
+        avg_stencil_radii[i] = s / (st.size() - 1.);
+    }
+}
+
+void Grid::writeStencilsToFile(const std::string& filename) const {
+    std::ofstream f(filename);
+    for (const StencilType& st : stencil_map) {
+        for (size_t j = 0; j < st.size(); j++) f << st[j] << (j + 1 < st.size() ? " " : "");
+        f << "\n";
+    }
+}
+
+int Grid::loadStencilsFromFile(const std::string& filename) {
+    std::ifstream f(filename);
+    if (!f.is_open()) return 1;
+    stencil_map.clear();
+    std::string line;
+    unsigned int mx = 0;
+    while (std::getline(f, line)) {
+        std::istringstream ss(line);
+        StencilType st;
+        int v;
+        while (ss >> v) st.push_back(v);
+        if (st.empty()) continue;
+        mx = st.size() > mx ? (unsigned)st.size() : mx;
+        stencil_map.push_back(st);
+    }
+    max_st_size = mx;
+    computeStencilRadii();
+    return 0;
+}
+
+void GridReader::generate() { readNodeList(extra_); }
+
+int GridReader::readNodeList(int expect_num_extra_dbls_per_line) {
+    std::ifstream fin(filename_);
+    if (!fin.is_open()) {
+        std::printf("Error opening node file to read\n");
+        std::exit(EXIT_FAILURE);
+    }
+    node_list.clear();
+    unsigned int i = 0;
+    while (fin.good()) {
+        double x, y, z, junk;
+        fin >> x >> y >> z;
+        for (int j = 0; j < expect_num_extra_dbls_per_line; j++) fin >> junk;
+        if (!fin.eof() && !fin.fail()) {
+            if (i < n_nodes_) {
+                node_list.push_back(NodeType(x, y, z));
+                i++;
+            } else {
+                break;
+            }
+        }
+    }
+    nb_nodes = (unsigned)node_list.size();
+    global_num_nodes = nb_nodes;
+    return 0;
+}
+
+}  // namespace rbfx

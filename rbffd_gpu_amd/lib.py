@@ -16,6 +16,7 @@ X, Y, Z, LAPL, R, HV, LAMBDA, THETA = 0x1, 0x2, 0x4, 0x8, 0x10, 0x20, 0x40, 0x80
 LSFC, XSFC, YSFC, ZSFC, INTERP = 0x100, 0x200, 0x400, 0x800, 0x8000
 F64, F32 = 0, 1
 SOLVE_F64_LU, SOLVE_F32_LU_F64_IR = 0, 1
+POLY_SPHERE = 16  # poly_deg flag: sphere-reduced monomial basis
 PATH_F64, PATH_F32_IR, PATH_F32_FALLBACK_F64, PATH_SINGULAR = 0, 1, 2, 3
 ERR_NAMES = {-1: "INVALID", -2: "HIP", -3: "NOMEM", -4: "NODEVICE", -5: "SINGULAR", -6: "STATE", -7: "TOO_LARGE"}
 
@@ -61,10 +62,12 @@ def load_library(path=None):
         "rbfx_h2d": (C.c_int, [vp, vp, vp, u64]),
         "rbfx_d2h": (C.c_int, [vp, vp, vp, u64]),
         "rbfx_d2d": (C.c_int, [vp, vp, vp, u64]),
+        "rbfx_stream_copy": (C.c_int, [vp, vp, vp, u64]),
         "rbfx_timer_start": (C.c_int, [vp]),
         "rbfx_timer_stop": (C.c_int, [vp, C.POINTER(f64)]),
         "rbfx_knn_kdtree": (C.c_int, [i32, vp, i32, vp, i32]),
         "rbfx_nodes_fibonacci": (C.c_int, [i32, f64, u64, vp]),
+        "rbfx_nodes_icosahedral": (C.c_int, [i32, vp, C.POINTER(i32)]),
         "rbfx_weights": (C.c_int, [vp, i32, vp, vp, i32, vp, vp, u32, i32, i32, i32, f64, i32, vp, vp, C.POINTER(i32)]),
         "rbfx_ell_create": (C.c_int, [vp, i32, i32, i32, vp, i32, vp, vp, i32, C.POINTER(vp)]),
         "rbfx_ell_destroy": (None, [vp]),
@@ -77,8 +80,8 @@ def load_library(path=None):
         "rbfx_rk4_get_state": (C.c_int, [vp, vp]),
         "rbfx_rk4_step": (C.c_int, [vp, f64, i32]),
         "rbfx_rk4_stage": (C.c_int, [vp, i32, f64]),
-        "rbfx_tune_set": (C.c_int, [vp, i32, i32, i32, i32, i32, i32]),
-        "rbfx_tune_get": (C.c_int, [vp, i32, i32, i32, i32, C.POINTER(i32), C.POINTER(i32)]),
+        "rbfx_tune_set": (C.c_int, [vp, i32, i32, i32, i32, i32, i32, i32]),
+        "rbfx_tune_get": (C.c_int, [vp, i32, i32, i32, i32, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)]),
         "rbfx_tune_clear": (C.c_int, []),
         "rbfx_cosine_bell_coefs": (C.c_int, [vp, i32, vp, f64, f64, f64, vp, vp]),
         "rbfx_cosine_bell_init": (C.c_int, [vp, i32, vp, f64, vp, i32]),
@@ -124,6 +127,18 @@ def nodes_fibonacci(n, jitter=1e-3, seed=0):
     rc = RBFX.rbfx_nodes_fibonacci(n, jitter, seed, _ptr(out))
     if rc:
         raise RbfxError(rc, "nodes_fibonacci failed")
+    return out
+
+
+def nodes_icosahedral(level):
+    n = C.c_int32()
+    rc = RBFX.rbfx_nodes_icosahedral(level, None, C.byref(n))
+    if rc:
+        raise RbfxError(rc, "nodes_icosahedral failed")
+    out = np.empty((n.value, 3), dtype=np.float64)
+    rc = RBFX.rbfx_nodes_icosahedral(level, _ptr(out), C.byref(n))
+    if rc:
+        raise RbfxError(rc, "nodes_icosahedral failed")
     return out
 
 
@@ -307,13 +322,14 @@ class EllOperators:
     def spmm(self, n_fields, d_u, d_out, field_type=F64, accum=F64):
         self.ctx.check(RBFX.rbfx_spmm(self.h, n_fields, d_u.ptr, d_out.ptr, field_type, accum), "spmm")
 
-    def tune_set(self, kind, n_fields, field_type, accum, fpt, block):
-        self.ctx.check(RBFX.rbfx_tune_set(self.h, kind, n_fields, field_type, accum, fpt, block), "tune_set")
+    def tune_set(self, kind, n_fields, field_type, accum, fpt, block, batch=4):
+        self.ctx.check(RBFX.rbfx_tune_set(self.h, kind, n_fields, field_type, accum, fpt, block, batch), "tune_set")
 
     def tune_get(self, kind, n_fields, field_type, accum):
-        f, b = C.c_int32(), C.c_int32()
-        self.ctx.check(RBFX.rbfx_tune_get(self.h, kind, n_fields, field_type, accum, C.byref(f), C.byref(b)), "tune_get")
-        return f.value, b.value
+        f, b, u = C.c_int32(), C.c_int32(), C.c_int32()
+        self.ctx.check(RBFX.rbfx_tune_get(self.h, kind, n_fields, field_type, accum, C.byref(f), C.byref(b), C.byref(u)),
+                       "tune_get")
+        return f.value, b.value, u.value
 
     def close(self):
         if self.h:

@@ -127,10 +127,12 @@ def test_spmm_tuned_shapes_agree(ctx):
     outs = []
     for fpt in (1, 2, 4, 8):
         for block in (64, 128, 256):
-            ell.tune_set(0, F, R.F64, R.F64, fpt, block)
-            d_o = ctx.alloc((4, n_rows, F), np.float64)
-            ell.spmm(F, d_u, d_o)
-            outs.append(d_o.download())
+            for batch in (4, 8):
+                ell.tune_set(0, F, R.F64, R.F64, fpt, block, batch)
+                d_o = ctx.alloc((4, n_rows, F), np.float64)
+                ell.spmm(F, d_u, d_o)
+                outs.append(d_o.download())
+                d_o.free()
     R.RBFX.rbfx_tune_clear()
     for o in outs[1:]:
         assert np.array_equal(o, outs[0])

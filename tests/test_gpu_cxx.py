@@ -1,0 +1,89 @@
+"""The C++ drop-in layer end to end (Grid -> RBFFD_HIP -> TimeDependentPDE_HIP / CosineBell_HIP),
+through its driver executables, checked against the CPU restatement."""
+import json
+import os
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle as O
+import rbffd_gpu_amd as R
+from conftest import ROOT
+
+pytestmark = pytest.mark.gpu
+BIN = os.path.join(ROOT, "rbffd_gpu_amd", "bin")
+U64 = 2.220446049250313e-16
+
+
+def run(args, cwd):
+    print("run:", " ".join(args), flush=True)
+    p = subprocess.run(args, cwd=cwd, capture_output=True, text=True, timeout=300)
+    assert p.returncode == 0, p.stdout[-2000:] + p.stderr[-2000:]
+    return p.stdout
+
+
+def test_derivative_driver_c1(tmp_path, c1):
+    xyz, st = c1["xyz"], c1["st"]
+    fn = tmp_path / "nodes.bin"
+    with open(fn, "wb") as f:
+        f.write(struct.pack("<q", xyz.shape[0]))
+        f.write(xyz.tobytes())
+    out = run([os.path.join(BIN, "rbfx_derivative_test"), str(fn), "31", "3.5", "2", str(tmp_path / "c1")], tmp_path)
+    res = json.loads(out.strip().splitlines()[-1])
+    N = 4096
+    data = np.fromfile(tmp_path / "c1.bin", dtype=np.float64)
+    derivs = data[:4 * N].reshape(4, N)
+    w_gpu = data[4 * N:].reshape(4, N, 31)
+    off, idx = R.csr_from_fixed(st)
+    x, y, z = xyz.T
+    f = np.exp(x) * np.sin(2 * y) + z ** 3
+    # 1) the derivatives the C++ layer returns are the GPU application of its weights (parity 1e-12)
+    for t in range(4):
+        ref = O.apply(off, idx, w_gpu[t].reshape(-1), f)
+        scale = (np.abs(w_gpu[t]) * np.abs(f[st])).sum(1)
+        assert np.max(np.abs(derivs[t] - ref) / scale) <= 1e-12
+    # 2) its weights agree with the CPU restatement's to the conditioning bound
+    w_ref, _, _ = O.weights(off, idx, xyz, 3.5, R.X | R.Y | R.Z | R.LSFC, dim=3, poly_deg=2)
+    rel = np.abs(w_gpu.reshape(4, -1) - w_ref).max() / np.abs(w_ref).max()
+    assert rel < 1e-4
+    assert res["max_backward_error"] < 64 * 41 * U64
+    # 3) MatrixMarket output round-trips (1-based, "%d %d %24.16le")
+    mtx = open(tmp_path / "c1_x.mtx").read().splitlines()
+    assert mtx[0].startswith("%%MatrixMarket")
+    assert mtx[1].split() == ["4096", "4096", str(4096 * 31)]
+    r0 = mtx[2].split()
+    assert int(r0[0]) == 1 and int(r0[1]) == st[0, 0] + 1 and float(r0[2]) == w_gpu[0, 0, 0]
+    print("analytic errors:", res["rel_err_vs_analytic"])
+
+
+def test_cosine_bell_driver_matches_cpu(tmp_path):
+    """Icosahedral level 4 (2562 nodes), n = 31 and the reference's test.conf parameters
+    (eps = 0.035 sqrt(N) - 0.1, HV k = 4, gamma = 5e-2, dim 2), one revolution in 500 RK4 steps
+    through CosineBell_HIP. The run is replayed on the CPU (restated CosineBell::solve +
+    advanceRungeKutta4) with the weights the GPU computed: the solutions agree to 1e-12; the
+    GPU weights agree with the CPU-solved weights to the conditioning bound."""
+    out = run([os.path.join(BIN, "rbfx_cosine_bell"), "ico:4", "31", "0.035", "0.1", "4", "0.05", "500", "1", "f64"],
+              tmp_path)
+    res = json.loads(out.strip().splitlines()[-1])
+    u_gpu = np.fromfile(tmp_path / "cosine_bell_solution.bin", dtype=np.float64)
+    xyz = R.nodes_icosahedral(4)
+    N = xyz.shape[0]
+    n = 31
+    w_gpu = np.fromfile(tmp_path / "cosine_bell_weights.bin", dtype=np.float64).reshape(3, N, n)
+    st = R.knn_kdtree(xyz, n)
+    off, idx = R.csr_from_fixed(st)
+    eps = 0.035 * np.sqrt(N) - 0.1
+    cl, ct = O.cosine_bell_coefs(xyz)
+    h0 = O.cosine_bell_exact(xyz)
+    u = O.rk4_step(st, [w_gpu[1], w_gpu[2], w_gpu[0]], [cl, ct, None], h0[:, None], 1036800.0 / 500, n_steps=500)[:, 0]
+    assert np.abs(u_gpu - u).max() <= 1e-12 * np.abs(u).max(), np.abs(u_gpu - u).max()
+    nr = O.sol_norms(u, h0)
+    assert abs(res["l2rel"] - nr["l2rel"]) <= 1e-10 * nr["l2rel"]
+    w_ref, _, _ = O.weights(off, idx, xyz, eps, R.LAMBDA | R.THETA | R.HV, dim=2, poly_deg=0, hv_k=4,
+                            hv_scale=-0.05 * N ** -4.0)
+    for t in range(3):
+        rel = np.abs(w_gpu[t].reshape(-1) - w_ref[t]).max() / np.abs(w_ref[t]).max()
+        assert rel < 1e-3, (t, rel)
+    print("cosine bell after one revolution:", {k: res[k] for k in ("l2rel", "linfrel")})

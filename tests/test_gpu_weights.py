@@ -107,6 +107,27 @@ def test_weights_f32_ir(ctx, c1):
     assert rel < 1e-4
 
 
+def test_weights_sphere_basis_matches_cpu(ctx):
+    """Nodes exactly on the sphere, degree 2 and 3 in the sphere-reduced basis, X/Y/Z/HV."""
+    xyz = R.nodes_icosahedral(4)
+    N = xyz.shape[0]
+    st = R.knn_kdtree(xyz, 50)
+    off, idx = R.csr_from_fixed(st)
+    mask = R.X | R.Y | R.Z | R.HV
+    for deg in (2 | R.POLY_SPHERE, 3 | R.POLY_SPHERE):
+        w_ref, dg_ref, ns = O.weights(off, idx, xyz, 4 * (0.044 * np.sqrt(N) - 0.14), mask, dim=2, poly_deg=deg, hv_k=4,
+                                      hv_scale=-2.0 ** -6 * N ** -4.0, diag=True)
+        w, dg = ctx.weights(off, idx, xyz, 4 * (0.044 * np.sqrt(N) - 0.14), mask, dim=2, poly_deg=deg, hv_k=4,
+                            hv_scale=-2.0 ** -6 * N ** -4.0, diag=True)
+        assert ns == 0
+        assert dg[:, 0].max() <= 64 * 66 * U64
+        rel = np.abs(w - w_ref).max(axis=1) / np.abs(w_ref).max(axis=1)
+        conds = [np.linalg.cond(O.stencil_system(xyz, st[i], 4 * (0.044 * np.sqrt(N) - 0.14), mask, 2, deg, 4)[0])
+                 for i in range(0, N, 97)]
+        print(f"  deg {deg}: forward rel diff per type {rel}, max cond {max(conds):.1e}")
+        assert rel.max() <= min(1.0, 64 * max(conds) * U64)
+
+
 def test_weights_singular_reported(ctx):
     """A duplicated node makes the system singular: reported, not silently propagated."""
     pts = np.array([[0, 0, 1], [0.1, 0, 0.99], [0.1, 0, 0.99], [0, 0.1, 0.99], [-0.1, 0, 0.99]], float)

@@ -1,0 +1,313 @@
+"""Pins the CPU restatement (oracle/) with checks that do not retype its formulas:
+closed forms, finite differences, library routines (LAPACK via scipy, Laguerre polynomials,
+sympy), known answers from the reference's own sources, and invariants."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+import rbffd_gpu_amd as R
+from conftest import GOLDEN
+
+U64 = 2.220446049250313e-16
+
+
+# ---- Gaussian RBF primitives (rbf_gaussian.h) -------------------------------------------------
+def test_gaussian_first_derivative_by_finite_differences():
+    eps = 1.7
+    rng = np.random.default_rng(0)
+    for _ in range(20):
+        d = rng.uniform(-0.8, 0.8, 3)
+        h = 1e-6
+        fd = (O.lib().orc_rbf_eval(eps, d[0] + h, d[1], d[2]) - O.lib().orc_rbf_eval(eps, d[0] - h, d[1], d[2])) / (2 * h)
+        assert abs(O.lib().orc_rbf_xderiv(eps, *d) - fd) < 1e-8
+
+
+def test_gaussian_laplacian_by_finite_differences():
+    eps = 1.3
+    rng = np.random.default_rng(1)
+    f = lambda p: O.lib().orc_rbf_eval(eps, *p)
+    h = 1e-4
+    for dim in (1, 2, 3):
+        for _ in range(10):
+            d = rng.uniform(-0.7, 0.7, 3)
+            lap = 0.0
+            for k in range(dim):
+                e = np.zeros(3)
+                e[k] = h
+                lap += (f(d + e) - 2 * f(d) + f(d - e)) / h ** 2
+            assert abs(O.lib().orc_rbf_lapl(eps, dim, *d) - lap) < 2e-6, (dim, d)
+
+
+def test_gaussian_radial_derivatives():
+    """phi'(r)/r and phi''(r) of phi(r) = exp(-(eps r)^2) by finite differences in r."""
+    eps = 2.1
+    for r in (0.05, 0.3, 0.9):
+        d = np.array([r, 0.0, 0.0])
+        phi = lambda s: np.exp(-(eps * s) ** 2)
+        h = 1e-5
+        d1 = (phi(r + h) - phi(r - h)) / (2 * h)
+        d2 = (phi(r + h) - 2 * phi(r) + phi(r - h)) / h ** 2
+        assert abs(O.lib().orc_rbf_rderiv_over_r(eps, *d) - d1 / r) < 1e-7
+        assert abs(O.lib().orc_rbf_rderiv2(eps, *d) - d2) < 1e-5
+
+
+def test_hyperviscosity_polynomial_is_generalised_laguerre():
+    """p_k(r) = (-4)^k k! L_k^{(d/2-1)}(eps^2 r^2), the identity stated at rbf_gaussian.h:193-199."""
+    from math import factorial
+
+    from scipy.special import eval_genlaguerre
+
+    for dim in (1, 2, 3):
+        for k in range(0, 7):
+            for eps, r2 in ((1.0, 0.3), (3.5, 0.01), (0.7, 2.0)):
+                want = (-4.0) ** k * factorial(k) * eval_genlaguerre(k, dim / 2.0 - 1.0, eps * eps * r2)
+                got = O.lib().orc_hv_pk(eps, dim, k, r2)
+                assert abs(got - want) <= 1e-9 * max(1.0, abs(want)), (dim, k, got, want)
+
+
+def test_hyperviscosity_equals_iterated_laplacian_symbolically():
+    """Delta^k exp(-eps^2 r^2) in d dimensions (radial Laplacian f'' + (d-1)/r f') by sympy."""
+    import sympy as sp
+
+    r, e = sp.symbols("r e", positive=True)
+    for dim in (2, 3):
+        f = sp.exp(-(e * r) ** 2)
+        for k in range(1, 4):
+            f = sp.simplify(sp.diff(f, r, 2) + (dim - 1) / r * sp.diff(f, r))
+            for ev, rv in ((1.5, 0.4), (3.0, 0.2)):
+                want = float(f.subs({e: ev, r: rv}))
+                got = O.lib().orc_rbf_hv(ev, dim, k, rv, 0.0, 0.0)
+                assert abs(got - want) <= 1e-10 * max(1.0, abs(want)), (dim, k)
+
+
+# ---- assembly and solve -----------------------------------------------------------------------
+def test_system_symmetry_and_solve_against_lapack_golden():
+    """The saddle-point matrix is exactly symmetric; the restated LU agrees with LAPACK
+    (scipy.linalg.solve) to the forward bound kappa * u on the golden systems."""
+    g = np.load(os.path.join(GOLDEN, "lapack_weights.npz"))
+    for q in range(int(g["count"])):
+        pts, meta, x = g[f"s{q}_xyz"], g[f"s{q}_meta"], g[f"s{q}_x"]
+        eps, mask, dim, deg, hvk, cond = meta[0], int(meta[1]), int(meta[2]), int(meta[3]), int(meta[4]), meta[5]
+        n = pts.shape[0]
+        M, Rr = O.stencil_system(pts, np.arange(n, dtype=np.int32), eps, mask, dim, deg, hvk)
+        assert np.array_equal(M, M.T)
+        X, info = O.lu_solve(M, Rr)
+        assert info == 0
+        rel = np.abs(X - x).max() / np.abs(x).max()
+        assert rel <= min(1.0, 64 * cond * U64), (q, rel, cond)
+        # backward error at the level of machine precision
+        r = Rr - M @ X
+        be = np.abs(r).max() / (np.abs(M).sum(1).max() * np.abs(X).max() + np.abs(Rr).max())
+        assert be < 64 * M.shape[0] * U64
+
+
+def test_lu_against_lapack_on_random_systems():
+    import scipy.linalg as sl
+
+    rng = np.random.default_rng(2)
+    for m in (1, 2, 7, 40, 121):
+        A = rng.standard_normal((m, m))
+        B = rng.standard_normal((m, 3))
+        X, info = O.lu_solve(A, B)
+        assert info == 0
+        assert np.allclose(X, sl.solve(A, B), rtol=1e-9, atol=1e-12)
+
+
+def test_lu_reports_exact_singularity():
+    A = np.array([[1.0, 2.0], [2.0, 4.0]])
+    _, info = O.lu_solve(A, np.ones((2, 1)))
+    assert info == 2
+
+
+def test_polynomial_reproduction_degree2():
+    """With degree-2 augmentation the X/Y/Z/LAPL weights differentiate quadratics exactly
+    (up to rounding amplified by the conditioning)."""
+    xyz = R.nodes_fibonacci(2000, 1e-3, 3)
+    st = R.knn_kdtree(xyz, 31)
+    off, idx = R.csr_from_fixed(st)
+    w, _, ns = O.weights(off, idx, xyz, 3.0, R.X | R.Y | R.Z | R.LAPL, dim=3, poly_deg=2)
+    assert ns == 0
+    w = w.reshape(4, 2000, 31)
+    for i in (0, 777, 1999):
+        d = xyz[st[i]] - xyz[i]
+        px, py, pz = d.T
+        scale = np.abs(w[:, i]).sum(1)
+        # constants -> 0
+        assert np.all(np.abs(w[:, i].sum(1)) < 1e-6 * scale)
+        # X applied to x-x0 -> 1, to y -> 0, to (x-x0)^2 -> 0 (at the centre); LAPL of |d|^2 -> 6
+        assert abs(w[0, i] @ px - 1) < 1e-6
+        assert abs(w[0, i] @ py) < 1e-6 * scale[0]
+        assert abs(w[0, i] @ (px * px)) < 1e-6 * scale[0]
+        assert abs(w[1, i] @ py - 1) < 1e-6
+        assert abs(w[2, i] @ pz - 1) < 1e-6
+        assert abs(w[3, i] @ (px * px + py * py + pz * pz) - 6) < 1e-5 * scale[3] / 100 + 1e-6
+
+
+def test_five_point_laplacian_flat_limit_kat():
+    """tests/unified/compute_weights.cxx:313-315: on an 11x11 grid with eps = 0.05, stencil 12
+    gives the 5-point Laplacian (-4,1,1,1,1)/h^2 = (-100,25,25,25,25)."""
+    g = np.linspace(-1, 1, 11)
+    xyz = np.array([[x, y, 0.0] for y in g for x in g])
+    st = R.knn_kdtree(xyz, 5)
+    off, idx = R.csr_from_fixed(st)
+    w, _, ns = O.weights(off, idx, xyz, 0.05, R.LAPL, dim=2, poly_deg=0)
+    ws = w[0].reshape(121, 5)[12]
+    order = np.argsort(-np.abs(ws))
+    assert abs(ws[order[0]] - (-100.0)) < 1e-2
+    assert np.allclose(ws[order[1:]], 25.0, atol=1e-2)
+    assert st[12][order[0]] == 12
+
+
+def test_laplace_beltrami_on_spherical_harmonics():
+    """Delta_S Y_lm = -l(l+1) Y_lm on nodes exactly on the unit sphere (icosahedral)."""
+    from bench import real_sph_harm
+
+    xyz = R.nodes_icosahedral(4)
+    n = 31
+    st = R.knn_kdtree(xyz, n)
+    off, idx = R.csr_from_fixed(st)
+    N = xyz.shape[0]
+    w, _, _ = O.weights(off, idx, xyz, 0.035 * np.sqrt(N) - 0.1, R.LSFC, dim=2, poly_deg=0)
+    for l, m in ((1, 0), (2, 1), (3, -2), (4, 3)):
+        Y = real_sph_harm(l, m, xyz)
+        d = O.apply(off, idx, w[0], Y)
+        err = np.abs(d + l * (l + 1) * Y).max() / (l * (l + 1) * np.abs(Y).max())
+        assert err < 2e-3, (l, m, err)
+
+
+def test_surface_gradient_on_a_linear_function():
+    """XSFC/YSFC/ZSFC of f = x on the unit sphere is (1 - x^2, -xy, -xz)."""
+    xyz = R.nodes_icosahedral(4)
+    st = R.knn_kdtree(xyz, 31)
+    off, idx = R.csr_from_fixed(st)
+    N = xyz.shape[0]
+    w, _, _ = O.weights(off, idx, xyz, 0.035 * np.sqrt(N) - 0.1, R.XSFC | R.YSFC | R.ZSFC, dim=3, poly_deg=0)
+    x, y, z = xyz.T
+    want = [1 - x * x, -x * y, -x * z]
+    for t in range(3):
+        assert np.abs(O.apply(off, idx, w[t], x) - want[t]).max() < 1e-3
+
+
+def test_lambda_theta_on_coordinate_functions():
+    """(1/cos theta) d/dlambda and d/dtheta of z and x on the sphere."""
+    xyz = R.nodes_icosahedral(4)
+    st = R.knn_kdtree(xyz, 31)
+    off, idx = R.csr_from_fixed(st)
+    N = xyz.shape[0]
+    w, _, _ = O.weights(off, idx, xyz, 0.035 * np.sqrt(N) - 0.1, R.LAMBDA | R.THETA, dim=2, poly_deg=0)
+    x, y, z = xyz.T
+    lam, th = np.arctan2(y, x), np.arctan2(z, np.hypot(x, y))
+    assert np.abs(O.apply(off, idx, w[0], z)).max() < 1e-3
+    assert np.abs(O.apply(off, idx, w[1], z) - np.cos(th)).max() < 1e-3
+    assert np.abs(O.apply(off, idx, w[0], x) + np.sin(lam)).max() < 1e-3
+    assert np.abs(O.apply(off, idx, w[1], x) + np.sin(th) * np.cos(lam)).max() < 1e-3
+
+
+# ---- apply, RK4, cosine bell, norms --------------------------------------------------------------
+def test_apply_integer_kat():
+    rng = np.random.default_rng(3)
+    idx = rng.integers(0, 50, size=(50, 7)).astype(np.int32)
+    w = rng.integers(-5, 6, size=(50, 7)).astype(float)
+    u = rng.integers(-9, 10, size=50).astype(float)
+    off, flat = R.csr_from_fixed(idx)
+    assert np.array_equal(O.apply(off, flat, w.reshape(-1), u), (w * u[idx]).sum(1))
+    out = O.spmm(idx, [w, 2 * w], np.stack([u, -u], 1))
+    assert np.array_equal(out[1, :, 1], -2 * (w * u[idx]).sum(1))
+
+
+def test_rk4_amplification_factor():
+    lam = np.linspace(-2.5, 0.8, 64)
+    u0 = np.linspace(1, 2, 64)
+    idx = np.arange(64, dtype=np.int32)[:, None]
+    dt = 0.2
+    u1 = O.rk4_step(idx, [lam[:, None]], None, u0[:, None], dt)[:, 0]
+    z = lam * dt
+    assert np.allclose(u1, u0 * (1 + z + z ** 2 / 2 + z ** 3 / 6 + z ** 4 / 24), rtol=1e-15, atol=0)
+
+
+def test_cosine_bell_exact_solution():
+    a = 1.0
+    c = np.array([[0.0, -1.0, 0.0]])  # lambda = 3pi/2, theta = 0: the bell centre
+    assert O.cosine_bell_exact(c, a)[0] == pytest.approx(1.0, abs=1e-15)
+    far = np.array([[0.0, 1.0, 0.0], [1.0, 0.0, 0.0], [0.0, 0.0, 1.0]])
+    assert np.all(O.cosine_bell_exact(far, a) == 0.0)
+    # half height at r = R/2 along the equator: h = (1 + cos(pi/2))/2 = 0.5
+    ang = 3 * np.pi / 2 + (a / 3) / 2
+    p = np.array([[np.cos(ang), np.sin(ang), 0.0]])
+    assert O.cosine_bell_exact(p, a)[0] == pytest.approx(0.5, abs=1e-12)
+
+
+def test_cosine_bell_velocity_is_solid_body_rotation():
+    """alpha = pi/2: the velocity field is rotation about the x axis with speed u0 = 2 pi a / T."""
+    xyz = R.nodes_icosahedral(3)
+    cl, ct = O.cosine_bell_coefs(xyz, 1.0, np.pi / 2, 1036800.0)
+    x, y, z = xyz.T
+    lam, th = np.arctan2(y, x), np.arctan2(z, np.hypot(x, y))
+    u0 = 2 * np.pi / 1036800.0
+    assert np.allclose(-cl, u0 * np.sin(th) * np.cos(lam), atol=1e-20)
+    assert np.allclose(-ct, -u0 * np.sin(lam), atol=1e-20)
+
+
+def test_norms_against_numpy():
+    rng = np.random.default_rng(5)
+    a, e = rng.standard_normal(100), rng.standard_normal(100)
+    n = O.sol_norms(a, e)
+    assert n["l1"] == pytest.approx(np.abs(a - e).sum())
+    assert n["l2"] == pytest.approx(np.linalg.norm(a - e))
+    assert n["linf"] == pytest.approx(np.abs(a - e).max())
+    assert n["l2rel"] == pytest.approx(np.linalg.norm(a - e) / np.linalg.norm(e))
+
+
+def test_planted_mistake_is_caught():
+    """A dropped term in the restated RHS (here: the LSFC phi'' term) must fail the
+    spherical-harmonic check above: the check is sensitive to the formula."""
+    from bench import real_sph_harm
+
+    xyz = R.nodes_icosahedral(4)
+    st = R.knn_kdtree(xyz, 31)
+    off, idx = R.csr_from_fixed(st)
+    N = xyz.shape[0]
+    eps = 0.035 * np.sqrt(N) - 0.1
+    # LSFC restated without its phi'' term equals 1/4 (4 - 3 r^2) phi'/r: build that "wrong"
+    # operator from the XSFC-independent pieces by solving with a modified right-hand side
+    M_ok = []
+    Y = real_sph_harm(2, 1, xyz)
+    good = O.apply(off, idx, O.weights(off, idx, xyz, eps, R.LSFC, dim=2)[0][0], Y)
+    wrong = np.empty(N)
+    for i in range(N):
+        M, Rr = O.stencil_system(xyz, st[i], eps, R.LSFC, 2, 0, 0)
+        d = xyz[i] - xyz[st[i]]
+        r2 = (d * d).sum(1)
+        Rr[:31, 0] = 0.25 * (4 - 3 * r2) * (-2 * eps * eps * np.exp(-eps * eps * r2))
+        X, _ = O.lu_solve(M, Rr)
+        wrong[i] = X[:31, 0] @ Y[st[i]]
+    err_good = np.abs(good + 6 * Y).max() / (6 * np.abs(Y).max())
+    err_wrong = np.abs(wrong + 6 * Y).max() / (6 * np.abs(Y).max())
+    assert err_good < 2e-3 and err_wrong > 1e-1
+    del M_ok
+
+
+def test_sphere_reduced_basis_reproduces_degree2_harmonics():
+    """On nodes exactly on the sphere the full degree-2 basis is singular; the 9-function
+    basis is not, and Laplace-Beltrami weights then reproduce Delta_S Y_lm = -l(l+1) Y_lm for
+    l <= 2 to rounding level (polynomial reproduction)."""
+    from bench import real_sph_harm
+
+    xyz = R.nodes_icosahedral(4)
+    st = R.knn_kdtree(xyz, 31)
+    off, idx = R.csr_from_fixed(st)
+    N = xyz.shape[0]
+    assert O.lib().orc_num_monomials(2 | R.POLY_SPHERE) == 9
+    assert O.lib().orc_num_monomials(3 | R.POLY_SPHERE) == 16
+    M, _ = O.stencil_system(xyz, st[100], 4.0, R.LSFC, 2, 2, 0)
+    assert np.linalg.cond(M) > 1e15  # full basis: singular on the sphere
+    M2, _ = O.stencil_system(xyz, st[100], 4.0, R.LSFC, 2, 2 | R.POLY_SPHERE, 0)
+    assert np.linalg.cond(M2) < 1e13
+    w, _, ns = O.weights(off, idx, xyz, 0.035 * np.sqrt(N) - 0.1, R.LSFC, dim=2, poly_deg=2 | R.POLY_SPHERE)
+    assert ns == 0
+    for l, m in ((1, 0), (2, -1), (2, 2)):
+        Y = real_sph_harm(l, m, xyz)
+        d = O.apply(off, idx, w[0], Y)
+        assert np.abs(d + l * (l + 1) * Y).max() < 1e-6 * l * (l + 1), (l, m)
