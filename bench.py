@@ -4,8 +4,8 @@ Workload (the fused multi-operator multi-field RK4 configuration):
   N = 1,000,000 nodes on the unit sphere (golden-angle spiral, +-1e-3 jitter, seed 0,
   projected back onto the sphere), n = 50 kd-tree stencils, Gaussian RBF
   eps = 4 (0.044 sqrt(N) - 0.14) (the reference's n=50 shape parameter is too flat for
-  stable advection on spiral nodes), degree-2 polynomials in the 9-function
-  basis that is independent on the sphere,
+  stable advection on spiral nodes), degree-2 polynomials in the 9-function basis that is
+  independent on the sphere,
   4 operators d/dx, d/dy, d/dz and hyperviscosity Delta^4 (dim 2, scale -2^-6 N^-4),
   16 fields = seeded combinations of real spherical harmonics of degree <= 20,
   PDE: solid-body rotation advection dq/dt = -(w x r).grad q + H q (one revolution per
@@ -69,7 +69,7 @@ def rotate(xyz, axis, angle):
 
 
 def bytes_per_stage(N, n, ops, F, s_w, s_f, n_coef, stage):
-    """Compulsory HBM bytes of one fused RK4 stage kernel launch (see DESIGN.md):
+    """Compulsory HBM bytes of one fused RK4 stage kernel launch:
     weights + indices once, the gathered stage field once per node, the FP64 state u,
     the FP64 accumulator (written in stage 1, read+written in 2-3, read in 4), the next
     stage field (stage 4 of the FP64 path writes u only), the coefficient arrays."""
@@ -85,6 +85,25 @@ def bytes_per_stage(N, n, ops, F, s_w, s_f, n_coef, stage):
     else:
         fld = s_f + 8 + 8 + 8 + (0 if f64 else s_f)
     return w + coef + F * N * fld
+
+
+STAGE_PROFILE = os.path.join(ROOT, "profiles", "rk4_stage_N1M_n50_ops4_F16.md")
+
+
+def profile_traffic(kernel):
+    """HBM bytes per launch of `kernel` measured by rocprofv3 FETCH_SIZE / WRITE_SIZE passes over
+    this same workload (scaled by the copy-kernel calibration), read from the committed profile
+    summary's traffic table; None when the profile holds no row for this kernel instance."""
+    try:
+        with open(STAGE_PROFILE) as fh:
+            for line in fh:
+                cells = [c.strip() for c in line.split("|")]
+                # traffic table rows: kernel | fetched | written | traffic | algorithmic | ratio | us | GB/s
+                if len(cells) == 10 and cells[1] == f"`{kernel}`":
+                    return float(cells[4])
+    except OSError:
+        pass
+    return None
 
 
 def stream_copy_gbs(ctx, R, nbytes=1 << 30, reps=10):
@@ -110,12 +129,13 @@ def main():
     ap.add_argument("--width", type=int, default=50)
     ap.add_argument("--fields", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-steps", type=int, default=0,
+                    help="CPU baseline RK4 steps (0: as many as fit in about 15 s after one probe step)")
     ap.add_argument("--kernel-only", action="store_true", help="set up, then launch only stage kernels (profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is designed "
-                                   "(DESIGN.md) but not built", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this build runs on one GPU only: the row-partitioned multi-GPU "
+                                   "step is designed but not built", "n_gpus": args.gpus}))
         sys.exit(2)
 
     import rbffd_gpu_amd as R
@@ -257,6 +277,8 @@ def main():
     avg_b32 = float(np.mean(bytes32))
     ach32 = avg_b32 / (avg_ms32 * 1e-3) / 1e9
     ach64 = float(np.mean(bytes64)) / (float(np.mean(stage_ms["fp64"])) * 1e-3) / 1e9
+    kname32 = f"rk4_stage_kernel<float, float, double, 4, {tuned['fp32']['fpt']}, {tuned['fp32']['batch']}>"
+    traffic32 = profile_traffic(kname32)
     log(f"stage kernel avg: fp32 {avg_ms32 * 1e3:.1f} us ({ach32:.0f} GB/s), "
         f"fp64 {np.mean(stage_ms['fp64']) * 1e3:.1f} us ({ach64:.0f} GB/s)")
 
@@ -283,15 +305,19 @@ def main():
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as O
 
-        nthreads = len(os.sched_getaffinity(0))
+        # the host's thread budget: OMP_NUM_THREADS when the environment sets it, else the affinity mask
+        nthreads = int(os.environ.get("OMP_NUM_THREADS") or len(os.sched_getaffinity(0)))
         w_host = [v.download() for v in w_views]
         coefs_h = [-vel[:, 0].copy(), -vel[:, 1].copy(), -vel[:, 2].copy(), None]
-        t = time.perf_counter()
-        O.rk4_step(st, w_host, coefs_h, u0, dt, n_steps=args.cpu_steps, n_threads=nthreads)
-        tc = time.perf_counter() - t
-        cpu = {"value": args.cpu_steps / tc, "unit": "RK4 steps/s", "cores": nthreads, "kind": "port",
-               "sample": f"{args.cpu_steps} FP64 RK4 step(s) of the same N={N}, n={n}, 4-operator, {F}-field "
-                         f"workload with the C++ restatement (oracle/oracle.cpp), OpenMP over rows"}
+        if args.cpu_steps > 0:
+            _, cpu_steps, tc = O.rk4_step(st, w_host, coefs_h, u0, dt, n_steps=args.cpu_steps, n_threads=nthreads,
+                                          time_budget=1e30)
+        else:  # bounded sample: whole steps until about 15 s of wall time have been used
+            _, cpu_steps, tc = O.rk4_step(st, w_host, coefs_h, u0, dt, n_steps=30, n_threads=nthreads,
+                                          time_budget=15.0)
+        cpu = {"value": round(cpu_steps / tc, 4), "unit": "RK4 steps/s", "cores": nthreads, "kind": "port",
+               "sample": f"{cpu_steps} FP64 RK4 steps ({tc:.1f} s) of the same N={N}, n={n}, 4-operator, "
+                         f"{F}-field workload with the C++ restatement (oracle/oracle.cpp), OpenMP over rows"}
         log(f"CPU baseline: {cpu}")
 
     value = K / wall32
@@ -310,11 +336,12 @@ def main():
         "data": "synthetic (seeded sphere nodes, spherical-harmonic fields)",
         "config": {"workload": f"fused RK4 step: N={N}, n={n} kd-tree stencils, 4 operators (X,Y,Z,HV Delta^4) x "
                                f"{F} fields, FP32-stored weights+fields, FP64 accumulation and state, hipGraph",
-                   "global_batch": N, "seq_len": n, "parallelism": "single GPU"},
+                   "N": N, "n": n, "operators": 4, "fields": F, "dt": dt, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(ach32, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(ach32 / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "rk4_stage_kernel<float,float,double,4,*>",
-                     "bytes_per_launch": int(avg_b32), "avg_launch_us": round(avg_ms32 * 1e3, 2)},
+                     "frac": round(ach32 / HBM_PEAK_GBS, 4), "traffic": traffic32,
+                     "kernel": kname32, "algorithmic_bytes_per_launch": int(avg_b32),
+                     "avg_launch_us": round(avg_ms32 * 1e3, 2),
+                     "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE, profiles/rk4_stage_N1M_n50_ops4_F16.md"},
         "cpu_baseline": cpu,
         "extra": {
             "fp64_steps_per_s": round(K / wall64, 2),

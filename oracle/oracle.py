@@ -135,7 +135,10 @@ def spmm(idx, ws, u, n_threads=0):
     return out
 
 
-def rk4_step(idx, ws, coefs, u, dt, n_steps=1, n_threads=0):
+def rk4_step(idx, ws, coefs, u, dt, n_steps=1, n_threads=0, time_budget=None):
+    """n_steps RK4 steps; with time_budget (seconds) also stop once the steps taken so far
+    have used it, and return (u, steps, seconds in the step loop) instead of u."""
+    import time
     idx = np.ascontiguousarray(idx, np.int32)
     n_rows, n = idx.shape
     W = np.ascontiguousarray(np.stack([np.asarray(w, np.float64).reshape(n_rows, n) for w in ws]))
@@ -143,9 +146,16 @@ def rk4_step(idx, ws, coefs, u, dt, n_steps=1, n_threads=0):
     F = 1 if u.ndim == 1 else u.shape[1]
     keep = [None if c is None else np.ascontiguousarray(c, np.float64) for c in (coefs or [None] * len(ws))]
     arr = (C.c_void_p * len(ws))(*[None if c is None else c.ctypes.data for c in keep])
-    for _ in range(n_steps):
+    t0 = time.perf_counter()
+    done = 0
+    while done < n_steps:
         lib().orc_rk4_step(n_rows, n, _p(idx), len(ws), _p(W), arr, F, dt, _p(u), n_threads)
-    return u
+        done += 1
+        if time_budget is not None and time.perf_counter() - t0 >= time_budget:
+            break
+    if time_budget is None:
+        return u
+    return u, done, time.perf_counter() - t0
 
 
 def cart2sph(x, y, z):
