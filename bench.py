@@ -6,9 +6,12 @@ Workload (the fused multi-operator multi-field RK4 configuration):
   eps = 4 (0.044 sqrt(N) - 0.14) (the reference's n=50 shape parameter is too flat for
   stable advection on spiral nodes), degree-2 polynomials in the 9-function basis that is
   independent on the sphere,
-  4 operators d/dx, d/dy, d/dz and hyperviscosity Delta^4 (dim 2, scale -2^-6 N^-4),
+  4 operators: d/dx, d/dy, d/dz on the sphere (the reference's XSFC/YSFC/ZSFC, the Cartesian
+  gradient projected onto the tangent plane; unprojected d/dx on nodes that all lie on the
+  sphere leaves the normal direction undetermined and its weights blow up) and
+  hyperviscosity Delta^4 (dim 2) scaled -gamma N^-4 with gamma = 2^7 per revolution,
   16 fields = seeded combinations of real spherical harmonics of degree <= 20,
-  PDE: solid-body rotation advection dq/dt = -(w x r).grad q + H q (one revolution per
+  PDE: solid-body rotation advection dq/dt = -(w x r).grad_S q + H q (one revolution per
   unit time), classic RK4, dt = 1/4000.
 One "step" = one RK4 step of all 16 fields (4 fused stage kernels, replayed from a hipGraph)
 with every input resident in HBM. Stencils and weights are built on the device before the
@@ -132,6 +135,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=0,
                     help="CPU baseline RK4 steps (0: as many as fit in about 15 s after one probe step)")
     ap.add_argument("--kernel-only", action="store_true", help="set up, then launch only stage kernels (profiling)")
+    ap.add_argument("--node-order", action="store_true",
+                    help="process rows in node order instead of the Morton locality order")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only: the row-partitioned multi-GPU "
@@ -151,9 +156,9 @@ def main():
     log("device", info)
 
     # ---- weights on the GPU ----------------------------------------------------------------
-    mask = R.X | R.Y | R.Z | R.HV
+    mask = R.XSFC | R.YSFC | R.ZSFC | R.HV  # columns in ascending bit order: HV, XSFC, YSFC, ZSFC
     eps = 4.0 * (0.044 * np.sqrt(N) - 0.14)
-    hv_k, hv_gamma = 4, 2.0 ** -6
+    hv_k, hv_gamma = 4, 2.0 ** 7
     hv_scale = -hv_gamma * float(N) ** -hv_k
     d_xyz = ctx.to_device(xyz)
     off, idx = R.csr_from_fixed(st)
@@ -172,15 +177,28 @@ def main():
     # ---- operators, fields, coefficients ------------------------------------------------------
     ell64 = R.EllOperators(ctx, d_st, w_views, storage=R.F64, types=R.dertype_bits(mask))
     ell32 = R.EllOperators(ctx, d_st, w_views, storage=R.F32, types=R.dertype_bits(mask))
-    ell_sp64 = R.EllOperators(ctx, d_st, [w_views[0]], storage=R.F64)
-    ell_sp32 = R.EllOperators(ctx, d_st, [w_views[0]], storage=R.F32)
+    ell_sp64 = R.EllOperators(ctx, d_st, [w_views[1]], storage=R.F64)  # surface d/dx
+    ell_sp32 = R.EllOperators(ctx, d_st, [w_views[1]], storage=R.F32)
+    if not args.node_order:
+        # rows processed along a Morton curve: a wave's rows share stencil nodes (results unchanged)
+        t0 = time.time()
+        order = R.order_morton(xyz)
+        for e in (ell64, ell32):
+            e.set_row_order(order)
+        log(f"Morton row order: {time.time() - t0:.2f}s")
+        t0 = time.time()
+        # node cap per tile: 28 KB of staged records per block (F fields of 4 or 8 bytes)
+        for name, e, fb in (("fp64", ell64, 8), ("fp32", ell32, 4)):
+            tl, k_max, n_tiles = e.build_local(64, (28 * 1024) // (F * fb))
+            log(f"LDS staging tables {name}: {n_tiles} tiles of <= {tl} rows, at most {k_max} distinct nodes per tile")
+        log(f"LDS staging tables: {time.time() - t0:.2f}s")
     t0 = time.time()
     u0, spec = make_fields(xyz, F, seed=0)
     log(f"fields: {time.time() - t0:.1f}s")
     axis = np.array([0.3, -0.2, 1.0])
     omega = 2 * np.pi * axis / np.linalg.norm(axis)  # one revolution per unit time
     vel = np.cross(omega, xyz)
-    coefs = [ctx.to_device(-vel[:, 0].copy()), ctx.to_device(-vel[:, 1].copy()), ctx.to_device(-vel[:, 2].copy()), None]
+    coefs = [None, ctx.to_device(-vel[:, 0].copy()), ctx.to_device(-vel[:, 1].copy()), ctx.to_device(-vel[:, 2].copy())]
     dt = 1.0 / 4000
     d_u0 = ctx.to_device(u0)
 
@@ -197,6 +215,7 @@ def main():
 
     rk32 = R.RK4(ctx, ell32, F, R.F32, R.F64, coefs=coefs)
     rk64 = R.RK4(ctx, ell64, F, R.F64, R.F64, coefs=coefs)
+    rk32a = R.RK4(ctx, ell32, F, R.F32, R.F32, coefs=coefs)  # FP32 accumulation of the gathers
 
     # ---- autotune the launch shapes (fields/thread, block, gather batch) ----------------------
     from rbffd_gpu_amd import tune as TU
@@ -204,11 +223,14 @@ def main():
     table = {}
     t0 = time.time()
     tuned = {}
-    for name, rk, ell, ft in (("fp32", rk32, ell32, R.F32), ("fp64", rk64, ell64, R.F64)):
+    for name, rk, ell, ft, acc in (("fp32", rk32, ell32, R.F32, R.F64), ("fp64", rk64, ell64, R.F64, R.F64),
+                                   ("fp32_acc32", rk32a, ell32, R.F32, R.F32)):
         rk.set_state(d_u0)
-        entry, res = TU.tune_rk4(ctx, rk, ell, ft, R.F64, dt=dt, table=table)
+        entry, res = TU.tune_rk4(ctx, rk, ell, ft, acc, dt=dt, table=table)
         tuned[name] = entry
         log(f"tuned {name} RK4 stage shape: {entry}; worst {res[-1][0] * 1e3:.0f} us")
+        log("  fastest shapes (us/step, fpt, block, batch, share): " +
+            "; ".join(f"{r[0] * 1e3:.0f} {r[1:]}" for r in res[:6]))
     for name, ell, ft, dt_np in (("spmv_fp64", ell_sp64, R.F64, np.float64), ("spmv_fp32", ell_sp32, R.F32, np.float32)):
         d_x1 = ctx.to_device(u0[:, 0].astype(dt_np))
         d_y1 = ctx.alloc((1, N, 1), dt_np)
@@ -241,11 +263,14 @@ def main():
     u32 = rk32.get_state()
     wall64, ev64 = timed(rk64, K, W)
     u64 = rk64.get_state()
+    wall32a, _ = timed(rk32a, K, W)
+    u32a = rk32a.get_state()
     log(f"FP32-stored: {K / wall32:.1f} steps/s (wall) {K / (ev32 / 1e3):.1f} (events); "
         f"FP64: {K / wall64:.1f} steps/s (wall) {K / (ev64 / 1e3):.1f} (events)")
 
     # deviation FP32 vs FP64 and error vs the analytic rotated field after W+K steps
     dev = float(np.abs(u32 - u64).max() / np.abs(u64).max())
+    dev_rms = float(np.sqrt(np.mean((u32 - u64) ** 2) / np.mean(u64 ** 2)))
     t_end = (W + K) * dt
     xr = rotate(xyz, omega, -np.linalg.norm(omega) * t_end)
     exact = np.zeros_like(u0)
@@ -254,7 +279,10 @@ def main():
             exact[:, f] += c * real_sph_harm(int(l), int(m), xr)
     err64 = float(np.abs(u64 - exact).max() / np.abs(exact).max())
     err32 = float(np.abs(u32 - exact).max() / np.abs(exact).max())
-    log(f"rel. deviation FP32 vs FP64 {dev:.2e}; max rel error vs analytic: FP64 {err64:.2e}, FP32 {err32:.2e}")
+    dev32a = float(np.abs(u32a - u64).max() / np.abs(u64).max())
+    dev32a_rms = float(np.sqrt(np.mean((u32a - u64) ** 2) / np.mean(u64 ** 2)))
+    log(f"FP32-stored with FP32 accumulation: {K / wall32a:.1f} steps/s, deviation {dev32a:.2e} (max) {dev32a_rms:.2e} (rms)")
+    log(f"rel. deviation FP32 vs FP64 {dev:.2e} (max), {dev_rms:.2e} (rms); max rel error vs analytic: FP64 {err64:.2e}, FP32 {err32:.2e}")
 
     # ---- dominant kernel: the fused stage kernel, timed per launch ---------------------------
     R_L = 50
@@ -277,7 +305,10 @@ def main():
     avg_b32 = float(np.mean(bytes32))
     ach32 = avg_b32 / (avg_ms32 * 1e-3) / 1e9
     ach64 = float(np.mean(bytes64)) / (float(np.mean(stage_ms["fp64"])) * 1e-3) / 1e9
-    kname32 = f"rk4_stage_kernel<float, float, double, 4, {tuned['fp32']['fpt']}, {tuned['fp32']['batch']}>"
+    t32 = tuned["fp32"]
+    sg32 = F // t32["fpt"] if t32.get("share") else 0
+    pd32 = t32["share"] if t32.get("share", 0) >= 2 else 0
+    kname32 = f"rk4_stage_kernel<float, float, double, 4, {t32['fpt']}, {t32['batch']}, {sg32}, {pd32}>"
     traffic32 = profile_traffic(kname32)
     log(f"stage kernel avg: fp32 {avg_ms32 * 1e3:.1f} us ({ach32:.0f} GB/s), "
         f"fp64 {np.mean(stage_ms['fp64']) * 1e3:.1f} us ({ach64:.0f} GB/s)")
@@ -308,7 +339,7 @@ def main():
         # the host's thread budget: OMP_NUM_THREADS when the environment sets it, else the affinity mask
         nthreads = int(os.environ.get("OMP_NUM_THREADS") or len(os.sched_getaffinity(0)))
         w_host = [v.download() for v in w_views]
-        coefs_h = [-vel[:, 0].copy(), -vel[:, 1].copy(), -vel[:, 2].copy(), None]
+        coefs_h = [None, -vel[:, 0].copy(), -vel[:, 1].copy(), -vel[:, 2].copy()]
         if args.cpu_steps > 0:
             _, cpu_steps, tc = O.rk4_step(st, w_host, coefs_h, u0, dt, n_steps=args.cpu_steps, n_threads=nthreads,
                                           time_budget=1e30)
@@ -334,7 +365,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32 storage / f64 accumulate+state",
         "data": "synthetic (seeded sphere nodes, spherical-harmonic fields)",
-        "config": {"workload": f"fused RK4 step: N={N}, n={n} kd-tree stencils, 4 operators (X,Y,Z,HV Delta^4) x "
+        "config": {"workload": f"fused RK4 step: N={N}, n={n} kd-tree stencils, 4 operators (surface d/dx,d/dy,d/dz + HV Delta^4) x "
                                f"{F} fields, FP32-stored weights+fields, FP64 accumulation and state, hipGraph",
                    "N": N, "n": n, "operators": 4, "fields": F, "dt": dt, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(ach32, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -345,8 +376,11 @@ def main():
         "cpu_baseline": cpu,
         "extra": {
             "fp64_steps_per_s": round(K / wall64, 2),
+            "fp32_acc32": {"steps_per_s": round(K / wall32a, 2), "rel_deviation_from_fp64": dev32a,
+                           "rms_rel_deviation_from_fp64": dev32a_rms},
             "fp32_over_fp64": round(wall64 / wall32, 3),
             "fp32_rel_deviation_from_fp64": dev,
+            "fp32_rms_rel_deviation_from_fp64": dev_rms,
             "rel_error_vs_analytic": {"fp64": err64, "fp32": err32, "t": t_end},
             "step_roofline_frac": {"fp32": round(sum(bytes32) / (wall32 / K) / 1e9 / HBM_PEAK_GBS, 4),
                                    "fp64": round(sum(bytes64) / (wall64 / K) / 1e9 / HBM_PEAK_GBS, 4)},

@@ -48,6 +48,10 @@ extern "C" {
 /* ---- precision selectors ------------------------------------------------------------- */
 #define RBFX_F64 0
 #define RBFX_F32 1
+/* Weight storage only (rbfx_ell_create): FP32 with plain round-to-nearest, the reference's
+ * (float) cast (src/rbffd/rbffd_cl.cpp:387-463). RBFX_F32 weight storage instead rounds each
+ * row so that the FP32 weights keep the FP64 row sum (each weight within 1.5 ulp). */
+#define RBFX_F32_RN 2
 
 /* weight-solve modes */
 #define RBFX_SOLVE_F64_LU 0          /* FP64 partial-pivot LU in LDS                       */
@@ -108,6 +112,10 @@ int rbfx_nodes_fibonacci(int32_t n_nodes, double jitter, uint64_t seed, double* 
  * N = 10*4^level + 2 (level 7 = 163842, the quasi-uniform cosine-bell grid size).
  * With xyz == NULL only *out_n is set. */
 int rbfx_nodes_icosahedral(int32_t level, double* xyz, int32_t* out_n);
+/* Host: order of the nodes along a 3-D Morton (Z-order) curve over their bounding box
+ * (21 bits per axis, ties by index): out_order[p] = node at position p. For
+ * rbfx_ell_set_row_order. No counterpart in the reference. */
+int rbfx_order_morton(int32_t n_nodes, const double* xyz, int32_t* out_order);
 
 /* ---- batched weight computation ------------------------------------------------------ */
 /* RBFFD::computeAllWeightsForAllStencils -> computeAllWeightsForStencil_Direct
@@ -141,12 +149,32 @@ int rbfx_weights(rbfx_ctx* ctx, int32_t n_stencils, const int32_t* d_st_off, con
  * RBFFD_CL::updateStencilsOnGPU/updateWeightsOnGPU (src/rbffd/rbffd_cl.cpp:215-257, 306-463)
  * and RBFFD_VCL's ell_matrix (src/rbffd/rbffd_vcl.cpp:224-299).
  * Inputs (device): d_idx n_rows*width int32 row-major (row i, slot j; pad slots = centre),
- * d_w[o] n_rows*width float64 row-major (pad weight 0); storage RBFX_F64 or RBFX_F32.
+ * d_w[o] n_rows*width float64 row-major (pad weight 0); storage RBFX_F64, RBFX_F32
+ * (row-sum-preserving rounding) or RBFX_F32_RN (nearest); both FP32 modes report RBFX_F32.
  * dertype_bits[o] (host) labels operator o. */
 int rbfx_ell_create(rbfx_ctx* ctx, int32_t n_rows, int32_t n_cols, int32_t width, const int32_t* d_idx,
                     int32_t n_ops, const uint32_t* dertype_bits, const double* const* d_w, int32_t storage,
                     rbfx_ell** out);
 void rbfx_ell_destroy(rbfx_ell* ell);
+/* Processing order of the rows (no counterpart in the reference, whose rows run in node order):
+ * d_order (device, n_rows int32, a permutation) lists the row handled at each position, and
+ * the operator set is re-laid out in that order. Results are unchanged bit for bit (every row
+ * is still computed the same way and written at its own index); what changes is which rows
+ * run side by side. With rows ordered along a space-filling curve (rbfx_order_morton), the
+ * rows of one wave share most of their stencil nodes, so the neighbour gathers hit in the
+ * CU's L1 instead of each going to L2. RK4 steppers on this set recapture their graph. */
+int rbfx_ell_set_row_order(rbfx_ell* ell, const int32_t* d_order);
+/* LDS-staged gathers (no counterpart in the reference): cuts the positions greedily into
+ * tiles of at most tile_rows (32, 64 or 128) consecutive rows with at most max_nodes distinct
+ * stencil nodes each (0 = no limit; at least the stencil width), and records per tile its
+ * node list and 16-bit local indices into it. The RK4 stage kernel's shape share = 4
+ * (rbfx_tune_set) then copies a tile's neighbour records into LDS once and gathers from
+ * there; max_nodes bounds that LDS (max_nodes * n_fields * field bytes per block). Build
+ * after rbfx_ell_set_row_order (re-ordering drops the tables); with rows in a locality order
+ * a 64-row tile has a few hundred distinct neighbours. rbfx_ell_local_info reports the tile
+ * size (0 = not built), the largest tile list and the number of tiles. */
+int rbfx_ell_build_local(rbfx_ell* ell, int32_t tile_rows, int32_t max_nodes);
+int rbfx_ell_local_info(const rbfx_ell* ell, int32_t* tile_rows, int32_t* k_max, int32_t* n_tiles);
 int rbfx_ell_info(const rbfx_ell* ell, int32_t* n_rows, int32_t* width, int32_t* n_ops, int32_t* storage);
 
 /* Derivative application RBFFD::applyWeightsForDeriv (src/rbffd/rbffd.cpp:669-688,
@@ -195,10 +223,16 @@ int rbfx_rk4_stage(rbfx_rk4* rk, int32_t stage, double dt);
  * (4 or 8 stencil slots per memory round trip) for this operator set's row count, width,
  * operator count, field count and precisions. rbfx_rk4_step picks the shape up when it
  * (re)captures its graph. No counterpart in the reference (its launch shape is fixed). */
+/* share = 1 (RK4 stage kernel only): the G = n_fields/fpt lanes of a row (G = 2 or 4, fpt > 1,
+ * batch divisible by G) split the index and weight loads of each slot batch and broadcast
+ * them inside the group, instead of every lane loading them. share = 2 or 3: the same, with
+ * those loads issued 2 or 3 batches ahead and the next batch's gathers issued before the
+ * current batch is consumed (3-4 operators; fpt 4 with 4 lanes or fpt 8 with 2 lanes, and
+ * for depth 2 also fpt 4 with 2 lanes or fpt 2 with 4 lanes). */
 int rbfx_tune_set(rbfx_ell* ell, int32_t kind, int32_t n_fields, int32_t field_type, int32_t accum, int32_t fpt,
-                  int32_t block, int32_t batch);
+                  int32_t block, int32_t batch, int32_t share);
 int rbfx_tune_get(rbfx_ell* ell, int32_t kind, int32_t n_fields, int32_t field_type, int32_t accum, int32_t* fpt,
-                  int32_t* block, int32_t* batch);
+                  int32_t* block, int32_t* batch, int32_t* share);
 int rbfx_tune_clear(void);
 
 /* ---- cosine bell pointwise terms (src/pdes/hyperbolic/cosine_bell.cpp:23-70) ---------- */

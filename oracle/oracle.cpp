@@ -569,4 +569,34 @@ void orc_sol_norms(int32_t n, const double* v, const double* ex, double* out) {
     out[5] = (lid > 1e-10) ? li / lid : (li + 1.) / (lid + 1.);
 }
 
+// This build's FP32 weight storage (no counterpart in the reference, whose FP32 mode is a
+// plain cast, rbffd_cl.cpp:387-463): nearest rounding, then one pass over the slots in
+// stencil order moving each weight by at most one FP32 ulp toward the remaining difference
+// between the FP64 row sum and the sum of the FP32 values.
+void orc_round_f32_rowsum(int32_t n_rows, int32_t width, const double* w, float* out) {
+    for (int i = 0; i < n_rows; i++) {
+        const double* src = w + (size_t)i * width;
+        double s64 = 0.0, s32 = 0.0;
+        for (int j = 0; j < width; j++) {
+            s64 += src[j];
+            s32 += (double)(float)src[j];
+        }
+        double r = s64 - s32;
+        for (int j = 0; j < width; j++) {
+            float v = (float)src[j];
+            if (v != 0.0f && std::isfinite(r)) {
+                float a = std::fabs(v);
+                double ulp = (double)std::nextafter(a, INFINITY) - (double)a;
+                double k = std::fmin(1.0, std::fmax(-1.0, std::rint(r / ulp)));
+                if (k != 0.0 && std::isfinite(ulp)) {
+                    float nv = (float)((double)v + k * ulp);
+                    r -= (double)nv - (double)v;
+                    v = nv;
+                }
+            }
+            out[(size_t)i * width + j] = v;
+        }
+    }
+}
+
 }  // extern "C"

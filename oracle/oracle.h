@@ -73,6 +73,9 @@ void orc_cosine_bell_rhs(int32_t n_rows, const int32_t* st_off, const int32_t* s
 /* PDE::calcSolNorms (src/pdes/pde.cpp:463-533): out = {l1abs, l1rel, l2abs, l2rel, liabs, lirel}. */
 void orc_sol_norms(int32_t n, const double* approx, const double* exact, double* out6);
 
+/* FP32 weight storage keeping the FP64 row sums (this build's rounding; row-major in/out). */
+void orc_round_f32_rowsum(int32_t n_rows, int32_t width, const double* w, float* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -60,6 +60,8 @@ def lib():
         L.orc_cosine_bell_rhs.argtypes = [i32, vp, vp, vp, vp, vp, vp, d, d, d, i32, vp, vp]
         L.orc_sol_norms.restype = None
         L.orc_sol_norms.argtypes = [i32, vp, vp, vp]
+        L.orc_round_f32_rowsum.restype = None
+        L.orc_round_f32_rowsum.argtypes = [i32, i32, vp, vp]
         _lib = L
     return _lib
 
@@ -192,6 +194,14 @@ def sol_norms(approx, exact):
     out = np.empty(6)
     lib().orc_sol_norms(a.size, _p(a), _p(e), _p(out))
     return dict(zip(("l1", "l1rel", "l2", "l2rel", "linf", "linfrel"), out))
+
+
+def round_f32_rowsum(w):
+    """FP32 copy of row-major weights (n_rows, width) that keeps the FP64 row sums."""
+    w = np.ascontiguousarray(w, np.float64)
+    out = np.empty(w.shape, np.float32)
+    lib().orc_round_f32_rowsum(w.shape[0], w.shape[1], _p(w), _p(out))
+    return out
 
 
 def ref_knn(xyz, k):

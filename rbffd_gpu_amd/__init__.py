@@ -14,10 +14,11 @@ from .lib import (  # noqa: F401
     load_library,
     dertype_bits,
     X, Y, Z, LAPL, R, HV, LAMBDA, THETA, LSFC, XSFC, YSFC, ZSFC, INTERP,
-    F64, F32,
+    F64, F32, F32_RN,
     SOLVE_F64_LU, SOLVE_F32_LU_F64_IR, POLY_SPHERE,
     knn_kdtree,
     nodes_fibonacci,
     nodes_icosahedral,
+    order_morton,
     csr_from_fixed,
 )

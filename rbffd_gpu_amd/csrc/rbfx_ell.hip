@@ -7,8 +7,10 @@
 // adjacent, several fields are gathered per neighbour, and nothing crosses PCIe per call.
 #include <algorithm>
 #include <cstring>
+#include <vector>
 
 #include "rbfx_ell_common.h"
+#include "rbfx_host_internal.h"
 
 namespace {
 
@@ -22,9 +24,13 @@ __global__ void ell_pack_kernel(const int32_t* __restrict__ idx, WPtrs w, int n_
     size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t total = (size_t)n_pad * width;
     if (tid >= total) return;
-    int j = (int)(tid / n_pad);
-    int row = (int)(tid % n_pad);
-    size_t dst = (size_t)j * n_pad + row;
+    // destination order: tile, slot, row in tile (consecutive threads write consecutive entries)
+    const size_t per_tile = (size_t)width * ELL_TILE;
+    const int tile = (int)(tid / per_tile);
+    const size_t rem = tid - (size_t)tile * per_tile;
+    const int j = (int)(rem / ELL_TILE);
+    const int row = tile * ELL_TILE + (int)(rem % ELL_TILE);
+    const size_t dst = tid;
     if (row < n_rows) {
         size_t src = (size_t)row * width + j;
         idx_t[dst] = idx[src];
@@ -32,6 +38,44 @@ __global__ void ell_pack_kernel(const int32_t* __restrict__ idx, WPtrs w, int n_
     } else {
         idx_t[dst] = 0;
         for (int o = 0; o < n_ops; o++) w_t[dst * n_ops + o] = (WT)0;
+    }
+}
+
+// FP32 storage with sum-preserving rounding, one thread per (row, operator): round each
+// weight to nearest, then walk the slots in stencil order (centre first, then by distance,
+// which is roughly descending magnitude) and move a weight by at most one FP32 ulp toward
+// the remaining difference between the FP64 row sum and the exact sum of the FP32 values.
+// Each stored weight stays within 1.5 ulp of its FP64 value, and the row sum of the stored
+// weights matches the FP64 row sum to about the ulp of the smallest weights instead of the
+// ulp of the largest. Derivative rows sum to zero, so the rounding no longer adds
+// (sum of the rounding errors) * u[centre] to every derivative: what is left scales with
+// the field's variation across the stencil, not with its magnitude.
+__global__ void ell_round_f32_kernel(WPtrs w, int n_ops, int n_rows, int width, float* __restrict__ w_t) {
+    size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= (size_t)n_rows * n_ops) return;
+    int row = (int)(tid / n_ops);
+    int o = (int)(tid - (size_t)row * n_ops);
+    const double* src = w.p[o] + (size_t)row * width;
+    double s64 = 0.0, s32 = 0.0;
+    for (int j = 0; j < width; j++) {
+        double v = src[j];
+        s64 += v;
+        s32 += (double)(float)v;
+    }
+    double r = s64 - s32;
+    for (int j = 0; j < width; j++) {
+        float v = (float)src[j];
+        if (v != 0.0f && std::isfinite(r)) {
+            float a = std::fabs(v);
+            double ulp = (double)nextafterf(a, INFINITY) - (double)a;
+            double k = std::fmin(1.0, std::fmax(-1.0, std::rint(r / ulp)));
+            if (k != 0.0 && std::isfinite(ulp)) {
+                float nv = (float)((double)v + k * ulp);
+                r -= (double)nv - (double)v;
+                v = nv;
+            }
+        }
+        w_t[(ell_row_base(row, width) + (size_t)j * ELL_TILE) * n_ops + o] = v;
     }
 }
 
@@ -44,8 +88,9 @@ __global__ void ell_validate_kernel(const int32_t* __restrict__ idx, size_t tota
 
 template <typename WT, typename UT, typename AT, int NOPS, int FPT, int UB>
 __global__ void __launch_bounds__(256) ell_spmm_kernel(const int32_t* __restrict__ idx_t, const WT* __restrict__ w_t,
-                                                       const UT* __restrict__ u, UT* __restrict__ out, int n_rows,
-                                                       int n_pad, int width, int F, int n_tiles, int nb) {
+                                                       const UT* __restrict__ u, UT* __restrict__ out,
+                                                       const int32_t* __restrict__ order, int n_rows, int width, int F,
+                                                       int n_tiles, int nb) {
     const int G = F / FPT;
     const int R = blockDim.x / G;
     int tile = xcd_tile(blockIdx.x, nb);
@@ -53,11 +98,12 @@ __global__ void __launch_bounds__(256) ell_spmm_kernel(const int32_t* __restrict
     int t = threadIdx.x;
     int rl = t / G;
     if (rl >= R) return;
-    int row = tile * R + rl;
-    if (row >= n_rows) return;
+    const int p = tile * R + rl;  // processing position (layout order)
+    if (p >= n_rows) return;
+    const int row = ell_row_at(order, p);
     int fofs = (t - rl * G) * FPT;
     AT acc[NOPS][FPT];
-    ell_gather_dot<WT, UT, AT, NOPS, FPT, UB>(idx_t, w_t, u, row, n_pad, width, F, fofs, acc);
+    ell_gather_dot<WT, UT, AT, NOPS, FPT, UB>(idx_t, w_t, u, p, width, F, fofs, acc);
 #pragma unroll
     for (int o = 0; o < NOPS; o++) {
         UT r[FPT];
@@ -71,19 +117,66 @@ __global__ void __launch_bounds__(256) ell_spmm_kernel(const int32_t* __restrict
 template <typename WT, typename UT, typename AT>
 __global__ void __launch_bounds__(256) ell_spmv_strided_kernel(const int32_t* __restrict__ idx_t,
                                                                const WT* __restrict__ w_t, const UT* __restrict__ u,
-                                                               UT* __restrict__ out, int n_rows, int n_pad, int width,
-                                                               int n_ops, int op, int n_tiles, int nb) {
+                                                               UT* __restrict__ out, const int32_t* __restrict__ order,
+                                                               int n_rows, int width, int n_ops, int op, int n_tiles,
+                                                               int nb) {
     int tile = xcd_tile(blockIdx.x, nb);
     if (tile >= n_tiles) return;
-    int row = tile * blockDim.x + threadIdx.x;
-    if (row >= n_rows) return;
+    const int p = tile * blockDim.x + threadIdx.x;
+    if (p >= n_rows) return;
+    const int row = ell_row_at(order, p);
     AT acc = 0;
+    const size_t base = ell_row_base(p, width);
     for (int j = 0; j < width; j++) {
-        size_t e = (size_t)j * n_pad + row;
+        size_t e = base + (size_t)j * ELL_TILE;
         int nb_ = idx_t[e];
         acc = fma_acc<AT>((AT)w_t[e * n_ops + op], (AT)u[nb_], acc);
     }
     out[row] = (UT)acc;
+}
+
+// Row-order validation: every row must appear exactly once. count[] starts zeroed.
+__global__ void ell_order_count_kernel(const int32_t* __restrict__ order, int n, int* __restrict__ count,
+                                       int* __restrict__ bad) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    int r = order[p];
+    if (r < 0 || r >= n) atomicAdd(bad, 1);
+    else atomicAdd(&count[r], 1);
+}
+
+__global__ void ell_order_check_kernel(const int* __restrict__ count, int n, int* __restrict__ bad) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < n && count[r] != 1) atomicAdd(bad, 1);
+}
+
+// old_pos[row] = layout position of row in the current layout.
+__global__ void ell_old_pos_kernel(const int32_t* __restrict__ old_order, int n, int32_t* __restrict__ old_pos) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n) old_pos[old_order ? old_order[p] : p] = p;
+}
+
+// New layout entry (position p, slot j) = old entry (old_pos[order[p]], j); pad positions keep
+// index 0 and weight 0. One thread per destination entry, in destination order.
+template <typename WT>
+__global__ void ell_reorder_kernel(const int32_t* __restrict__ idx_o, const WT* __restrict__ w_o,
+                                   const int32_t* __restrict__ order, const int32_t* __restrict__ old_pos, int n_rows,
+                                   int n_pad, int width, int n_ops, int32_t* __restrict__ idx_n, WT* __restrict__ w_n) {
+    size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= (size_t)n_pad * width) return;
+    const size_t per_tile = (size_t)width * ELL_TILE;
+    const int tile = (int)(tid / per_tile);
+    const size_t rem = tid - (size_t)tile * per_tile;
+    const int j = (int)(rem / ELL_TILE);
+    const int p = tile * ELL_TILE + (int)(rem % ELL_TILE);
+    if (p < n_rows) {
+        size_t src = ell_row_base(old_pos[order[p]], width) + (size_t)j * ELL_TILE;
+        idx_n[tid] = idx_o[src];
+        for (int o = 0; o < n_ops; o++) w_n[tid * n_ops + o] = w_o[src * n_ops + o];
+    } else {
+        idx_n[tid] = 0;
+        for (int o = 0; o < n_ops; o++) w_n[tid * n_ops + o] = (WT)0;
+    }
 }
 
 inline int grid_blocks(int n_tiles) { return (n_tiles + 7) / 8 * 8; }
@@ -103,7 +196,7 @@ int launch_spmm_fpt(rbfx_ell* e, const EllTune& t, int F, const void* u, void* o
     hipStream_t s = e->ctx->stream;
 #define RBFX_SPMM_CASE(FPT_, UB_)                                                                                   \
     if (fpt == FPT_ && t.ub == UB_) {                                                                               \
-        ell_spmm_kernel<WT, UT, AT, NOPS, FPT_, UB_><<<nb, block, 0, s>>>(it, wt, uu, oo, e->n_rows, e->n_pad,      \
+        ell_spmm_kernel<WT, UT, AT, NOPS, FPT_, UB_><<<nb, block, 0, s>>>(it, wt, uu, oo, e->order, e->n_rows,     \
                                                                            e->width, F, n_tiles, nb);               \
         RBFX_LAUNCH_CHECK(e->ctx, "ell_spmm_kernel");                                                              \
         return RBFX_OK;                                                                                             \
@@ -149,7 +242,10 @@ int rbfx_ell_create(rbfx_ctx* ctx, int32_t n_rows, int32_t n_cols, int32_t width
     *out = nullptr;
     if (n_rows <= 0 || n_cols <= 0 || width <= 0 || !d_idx || n_ops <= 0 || n_ops > 16 || !d_w)
         return rbfx_fail(ctx, RBFX_E_INVALID, "ell_create: bad sizes (rows %d cols %d width %d ops %d)", n_rows, n_cols, width, n_ops);
-    if (storage != RBFX_F64 && storage != RBFX_F32) return rbfx_fail(ctx, RBFX_E_INVALID, "ell_create: bad storage %d", storage);
+    if (storage != RBFX_F64 && storage != RBFX_F32 && storage != RBFX_F32_RN)
+        return rbfx_fail(ctx, RBFX_E_INVALID, "ell_create: bad storage %d", storage);
+    const bool sum_preserving = storage == RBFX_F32;
+    if (storage == RBFX_F32_RN) storage = RBFX_F32;
     for (int o = 0; o < n_ops; o++)
         if (!d_w[o]) return rbfx_fail(ctx, RBFX_E_INVALID, "ell_create: weight pointer %d is NULL", o);
     RBFX_HIP_CHECK(ctx, hipSetDevice(ctx->device));
@@ -196,6 +292,12 @@ int rbfx_ell_create(rbfx_ctx* ctx, int32_t n_rows, int32_t n_cols, int32_t width
     else
         ell_pack_kernel<float><<<blocks, 256, 0, ctx->stream>>>(d_idx, wp, n_ops, n_rows, e->n_pad, width, e->idx_t, (float*)e->w_t);
     err = hipGetLastError();
+    if (err == hipSuccess && sum_preserving) {  // overwrites the real rows' nearest-rounded weights
+        size_t pairs = (size_t)n_rows * n_ops;
+        ell_round_f32_kernel<<<(unsigned)((pairs + 255) / 256), 256, 0, ctx->stream>>>(wp, n_ops, n_rows, width,
+                                                                                        (float*)e->w_t);
+        err = hipGetLastError();
+    }
     if (err == hipSuccess) err = hipStreamSynchronize(ctx->stream);
     if (err != hipSuccess) {
         rbfx_ell_destroy(e);
@@ -205,13 +307,164 @@ int rbfx_ell_create(rbfx_ctx* ctx, int32_t n_rows, int32_t n_cols, int32_t width
     return RBFX_OK;
 }
 
+static void free_local(rbfx_ell* e) {
+    hipFree(e->loc_pos);
+    hipFree(e->loc_off);
+    hipFree(e->loc_nodes);
+    hipFree(e->loc_idx);
+    e->loc_pos = e->loc_off = e->loc_nodes = nullptr;
+    e->loc_idx = nullptr;
+    e->lt = e->k_max = e->n_ltiles = 0;
+}
+
 void rbfx_ell_destroy(rbfx_ell* e) {
     if (!e) return;
     hipSetDevice(e->ctx->device);
     hipStreamSynchronize(e->ctx->stream);
     hipFree(e->idx_t);
     hipFree(e->w_t);
+    hipFree(e->order);
+    free_local(e);
     delete e;
+}
+
+int rbfx_ell_build_local(rbfx_ell* e, int32_t tile_rows, int32_t max_nodes) {
+    if (!e) return RBFX_E_INVALID;
+    rbfx_ctx* ctx = e->ctx;
+    if (tile_rows != 32 && tile_rows != 64 && tile_rows != 128)
+        return rbfx_fail(ctx, RBFX_E_INVALID, "ell_build_local: tile_rows %d not 32, 64 or 128", tile_rows);
+    if (max_nodes < 0 || (max_nodes > 0 && max_nodes < e->width))
+        return rbfx_fail(ctx, RBFX_E_INVALID, "ell_build_local: max_nodes %d below the stencil width %d", max_nodes,
+                         e->width);
+    RBFX_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    const size_t entries = (size_t)e->n_pad * e->width;
+    std::vector<int32_t> idx(entries);
+    RBFX_HIP_CHECK(ctx, hipMemcpyAsync(idx.data(), e->idx_t, entries * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+    RBFX_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    std::vector<int32_t> pos, off, nodes;
+    std::vector<uint16_t> lidx;
+    const int k_max = rbfx_build_local_tables(idx.data(), e->n_rows, e->n_pad, e->width, ELL_TILE, tile_rows, max_nodes,
+                                              pos, off, nodes, lidx);
+    if (k_max < 0)
+        return rbfx_fail(ctx, RBFX_E_TOO_LARGE, "ell_build_local: a tile of %d rows has more than 65535 distinct nodes", tile_rows);
+    int lds_attr = 0;
+    if (hipDeviceGetAttribute(&lds_attr, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device) != hipSuccess || lds_attr <= 0)
+        lds_attr = 64 * 1024;
+    int32_t *d_pos = nullptr, *d_off = nullptr, *d_nodes = nullptr;
+    uint16_t* d_lidx = nullptr;
+    hipError_t err;
+    auto drop = [&]() {
+        hipFree(d_pos);
+        hipFree(d_off);
+        hipFree(d_nodes);
+        hipFree(d_lidx);
+    };
+    if ((err = hipMalloc(&d_pos, pos.size() * sizeof(int32_t))) != hipSuccess ||
+        (err = hipMalloc(&d_off, off.size() * sizeof(int32_t))) != hipSuccess ||
+        (err = hipMalloc(&d_nodes, std::max<size_t>(1, nodes.size()) * sizeof(int32_t))) != hipSuccess ||
+        (err = hipMalloc(&d_lidx, lidx.size() * sizeof(uint16_t))) != hipSuccess) {
+        drop();
+        return rbfx_fail(ctx, RBFX_E_NOMEM, "ell_build_local: %s", hipGetErrorString(err));
+    }
+    hipMemcpyAsync(d_pos, pos.data(), pos.size() * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream);
+    hipMemcpyAsync(d_off, off.data(), off.size() * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream);
+    hipMemcpyAsync(d_nodes, nodes.data(), nodes.size() * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream);
+    hipMemcpyAsync(d_lidx, lidx.data(), lidx.size() * sizeof(uint16_t), hipMemcpyHostToDevice, ctx->stream);
+    if ((err = hipStreamSynchronize(ctx->stream)) != hipSuccess) {
+        drop();
+        return rbfx_fail(ctx, RBFX_E_HIP, "ell_build_local: %s", hipGetErrorString(err));
+    }
+    free_local(e);
+    e->loc_pos = d_pos;
+    e->n_ltiles = (int)pos.size() - 1;
+    e->loc_off = d_off;
+    e->loc_nodes = d_nodes;
+    e->loc_idx = d_lidx;
+    e->lt = tile_rows;
+    e->k_max = k_max;
+    e->lds_limit = lds_attr;
+    e->layout_version++;
+    return RBFX_OK;
+}
+
+int rbfx_ell_local_info(const rbfx_ell* e, int32_t* tile_rows, int32_t* k_max, int32_t* n_tiles) {
+    if (!e) return RBFX_E_INVALID;
+    if (tile_rows) *tile_rows = e->lt;
+    if (k_max) *k_max = e->k_max;
+    if (n_tiles) *n_tiles = e->n_ltiles;
+    return RBFX_OK;
+}
+
+int rbfx_ell_set_row_order(rbfx_ell* e, const int32_t* d_order) {
+    if (!e) return RBFX_E_INVALID;
+    rbfx_ctx* ctx = e->ctx;
+    if (!d_order) return rbfx_fail(ctx, RBFX_E_INVALID, "ell_set_row_order: order is NULL");
+    RBFX_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    const int n = e->n_rows;
+    const size_t entries = (size_t)e->n_pad * e->width;
+    const size_t wsz = e->storage == RBFX_F64 ? sizeof(double) : sizeof(float);
+    hipStream_t s = ctx->stream;
+    int* d_count = nullptr;
+    int32_t *d_old_pos = nullptr, *d_new_order = nullptr, *idx_n = nullptr;
+    void* w_n = nullptr;
+    hipError_t err;
+    auto release = [&]() {
+        hipStreamSynchronize(s);
+        hipFree(d_count);
+        hipFree(d_old_pos);
+        hipFree(d_new_order);
+        hipFree(idx_n);
+        hipFree(w_n);
+    };
+    if ((err = hipMalloc(&d_count, sizeof(int) * ((size_t)n + 1))) != hipSuccess ||
+        (err = hipMalloc(&d_old_pos, sizeof(int32_t) * (size_t)n)) != hipSuccess ||
+        (err = hipMalloc(&d_new_order, sizeof(int32_t) * (size_t)n)) != hipSuccess ||
+        (err = hipMalloc(&idx_n, sizeof(int32_t) * entries)) != hipSuccess ||
+        (err = hipMalloc(&w_n, wsz * entries * e->n_ops)) != hipSuccess) {
+        release();
+        return rbfx_fail(ctx, RBFX_E_NOMEM, "ell_set_row_order: allocation failed: %s", hipGetErrorString(err));
+    }
+    // validate: a permutation of 0..n-1 (count[n] collects out-of-range entries and repeats)
+    int bad = 0;
+    const unsigned g = (unsigned)((n + 255) / 256);
+    hipMemsetAsync(d_count, 0, sizeof(int) * ((size_t)n + 1), s);
+    ell_order_count_kernel<<<g, 256, 0, s>>>(d_order, n, d_count, d_count + n);
+    ell_order_check_kernel<<<g, 256, 0, s>>>(d_count, n, d_count + n);
+    hipMemcpyAsync(&bad, d_count + n, sizeof(int), hipMemcpyDeviceToHost, s);
+    if ((err = hipStreamSynchronize(s)) != hipSuccess) {
+        release();
+        return rbfx_fail(ctx, RBFX_E_HIP, "ell_set_row_order: %s", hipGetErrorString(err));
+    }
+    if (bad) {
+        release();
+        return rbfx_fail(ctx, RBFX_E_INVALID, "ell_set_row_order: not a permutation of the %d rows", n);
+    }
+    hipMemcpyAsync(d_new_order, d_order, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToDevice, s);
+    ell_old_pos_kernel<<<g, 256, 0, s>>>(e->order, n, d_old_pos);
+    const unsigned ge = (unsigned)((entries + 255) / 256);
+    if (e->storage == RBFX_F64)
+        ell_reorder_kernel<double><<<ge, 256, 0, s>>>(e->idx_t, (const double*)e->w_t, d_new_order, d_old_pos, n, e->n_pad,
+                                                      e->width, e->n_ops, idx_n, (double*)w_n);
+    else
+        ell_reorder_kernel<float><<<ge, 256, 0, s>>>(e->idx_t, (const float*)e->w_t, d_new_order, d_old_pos, n, e->n_pad,
+                                                     e->width, e->n_ops, idx_n, (float*)w_n);
+    if ((err = hipGetLastError()) != hipSuccess || (err = hipStreamSynchronize(s)) != hipSuccess) {
+        release();
+        return rbfx_fail(ctx, RBFX_E_HIP, "ell_set_row_order: reorder failed: %s", hipGetErrorString(err));
+    }
+    hipFree(e->idx_t);
+    hipFree(e->w_t);
+    hipFree(e->order);
+    e->idx_t = idx_n;
+    e->w_t = w_n;
+    e->order = d_new_order;
+    free_local(e);  // the local tables describe the previous layout
+    e->layout_version++;
+    idx_n = nullptr;
+    w_n = nullptr;
+    d_new_order = nullptr;
+    release();
+    return RBFX_OK;
 }
 
 int rbfx_ell_info(const rbfx_ell* e, int32_t* n_rows, int32_t* width, int32_t* n_ops, int32_t* storage) {
@@ -256,13 +509,13 @@ int rbfx_spmv(rbfx_ell* e, int32_t op, const void* d_u, void* d_out, int32_t fie
     int nb = grid_blocks(n_tiles);
     hipStream_t s = e->ctx->stream;
     if (e->storage == RBFX_F64)
-        ell_spmv_strided_kernel<double, double, double><<<nb, block, 0, s>>>(e->idx_t, (const double*)e->w_t, (const double*)d_u, (double*)d_out, e->n_rows, e->n_pad, e->width, e->n_ops, op, n_tiles, nb);
+        ell_spmv_strided_kernel<double, double, double><<<nb, block, 0, s>>>(e->idx_t, (const double*)e->w_t, (const double*)d_u, (double*)d_out, e->order, e->n_rows, e->width, e->n_ops, op, n_tiles, nb);
     else if (field_type == RBFX_F32 && accum == RBFX_F64)
-        ell_spmv_strided_kernel<float, float, double><<<nb, block, 0, s>>>(e->idx_t, (const float*)e->w_t, (const float*)d_u, (float*)d_out, e->n_rows, e->n_pad, e->width, e->n_ops, op, n_tiles, nb);
+        ell_spmv_strided_kernel<float, float, double><<<nb, block, 0, s>>>(e->idx_t, (const float*)e->w_t, (const float*)d_u, (float*)d_out, e->order, e->n_rows, e->width, e->n_ops, op, n_tiles, nb);
     else if (field_type == RBFX_F32)
-        ell_spmv_strided_kernel<float, float, float><<<nb, block, 0, s>>>(e->idx_t, (const float*)e->w_t, (const float*)d_u, (float*)d_out, e->n_rows, e->n_pad, e->width, e->n_ops, op, n_tiles, nb);
+        ell_spmv_strided_kernel<float, float, float><<<nb, block, 0, s>>>(e->idx_t, (const float*)e->w_t, (const float*)d_u, (float*)d_out, e->order, e->n_rows, e->width, e->n_ops, op, n_tiles, nb);
     else
-        ell_spmv_strided_kernel<float, double, double><<<nb, block, 0, s>>>(e->idx_t, (const float*)e->w_t, (const double*)d_u, (double*)d_out, e->n_rows, e->n_pad, e->width, e->n_ops, op, n_tiles, nb);
+        ell_spmv_strided_kernel<float, double, double><<<nb, block, 0, s>>>(e->idx_t, (const float*)e->w_t, (const double*)d_u, (double*)d_out, e->order, e->n_rows, e->width, e->n_ops, op, n_tiles, nb);
     RBFX_LAUNCH_CHECK(e->ctx, "ell_spmv_strided_kernel");
     return RBFX_OK;
 }

@@ -1,17 +1,29 @@
 // Device layout of a fused ELL operator set and the gather-dot core shared by the SpMV,
 // SpMM and RK4 stage kernels.
 //
-// Layout in HBM (n_pad = n_rows rounded up to ELL_ROW_PAD):
-//   idx_t[j * n_pad + row]                 int32, slot-major: one coalesced load per slot
-//   w_t[(j * n_pad + row) * n_ops + o]     weights of the n_ops operators of one (slot,row)
-//                                          side by side, so one vector load fetches all of
-//                                          them; FP64 or FP32 storage
+// Layout in HBM: rows in tiles of ELL_TILE (n_pad = n_rows rounded up to a whole tile),
+// slot-major inside a tile. With e = (tile * width + j) * ELL_TILE + (row % ELL_TILE):
+//   idx_t[e]                 int32: slot j of consecutive rows is one coalesced load
+//   w_t[e * n_ops + o]       weights of the n_ops operators of one (slot,row) side by side,
+//                            so one vector load fetches all of them; FP64 or FP32 storage
+// All slots of a row lie within one tile's width * ELL_TILE entries (200 KB of FP32 weights
+// at n = 50, 4 operators): a wave's loads over its whole stencil stay inside one or two
+// pages, where a plain slot-major layout (slot stride = all rows) puts every slot on a page
+// of its own and keeps the per-CU address-translation cache missing.
 // Fields are node-major u[node * F + f], so the gathered neighbour record of FPT fields is
 // one contiguous vector load. Pad rows carry index 0 and weight 0.
 #pragma once
+#include <type_traits>
+
 #include "rbfx_internal.h"
 
-#define ELL_ROW_PAD 256
+#define ELL_TILE 256
+#define ELL_ROW_PAD ELL_TILE
+
+// Element offset of slot 0 of `row` in the tiled layout; slot j adds j * ELL_TILE.
+__host__ __device__ __forceinline__ size_t ell_row_base(int row, int width) {
+    return ((size_t)(row / ELL_TILE) * (size_t)width) * ELL_TILE + (size_t)(row % ELL_TILE);
+}
 
 struct rbfx_ell {
     rbfx_ctx* ctx = nullptr;
@@ -20,7 +32,21 @@ struct rbfx_ell {
     uint32_t types[16] = {0};
     int32_t* idx_t = nullptr;  // device
     void* w_t = nullptr;       // device, double or float
+    int32_t* order = nullptr;  // device, optional: row processed at position p (layout position p)
+    int layout_version = 0;    // bumped whenever idx_t / w_t / order are replaced
+    // LDS staging (optional): positions cut into n_ltiles tiles of at most lt rows (tile t =
+    // positions loc_pos[t] .. loc_pos[t+1]); tile t gathers only from its distinct neighbour
+    // nodes loc_nodes[loc_off[t] .. loc_off[t+1]) (at most k_max), which a block copies into
+    // LDS once; loc_idx (same layout as idx_t) indexes that list.
+    int lt = 0, k_max = 0, lds_limit = 0, n_ltiles = 0;
+    int32_t* loc_pos = nullptr;
+    int32_t* loc_off = nullptr;
+    int32_t* loc_nodes = nullptr;
+    uint16_t* loc_idx = nullptr;
 };
+
+// Row handled at processing position p (the identity unless a locality order is set).
+__device__ __forceinline__ int ell_row_at(const int32_t* __restrict__ order, int p) { return order ? order[p] : p; }
 
 template <typename T, int N>
 struct alignas(((N * sizeof(T)) & (N * sizeof(T) - 1)) == 0 && N * sizeof(T) <= 16 ? N * sizeof(T) : sizeof(T)) Pack {
@@ -98,22 +124,23 @@ __device__ __forceinline__ float fma_acc<float>(float a, float b, float c) { ret
 // accumulator stays j = 0, 1, ..., width-1 whatever UB is.
 template <typename WT, typename UT, typename AT, int NOPS, int FPT, int UB>
 __device__ __forceinline__ void ell_gather_dot(const int32_t* __restrict__ idx_t, const WT* __restrict__ w_t,
-                                               const UT* __restrict__ u, int row, int n_pad, int width, int F,
+                                               const UT* __restrict__ u, int row, int width, int F,
                                                int fofs, AT (&acc)[NOPS][FPT]) {
 #pragma unroll
     for (int o = 0; o < NOPS; o++)
 #pragma unroll
         for (int f = 0; f < FPT; f++) acc[o][f] = (AT)0;
-    const int32_t* ip = idx_t + row;
-    const WT* wp = w_t + (size_t)row * NOPS;
-    const size_t wstride = (size_t)n_pad * NOPS;
+    const size_t base = ell_row_base(row, width);
+    const int32_t* ip = idx_t + base;
+    const WT* wp = w_t + base * NOPS;
+    const size_t wstride = (size_t)ELL_TILE * NOPS;
     const UT* ub = u + fofs;
     int nb[UB];
     WT w[UB][NOPS];
 #pragma unroll
     for (int q = 0; q < UB; q++) {
         if (q < width) {
-            nb[q] = __builtin_nontemporal_load(ip + (size_t)q * n_pad);
+            nb[q] = __builtin_nontemporal_load(ip + (size_t)q * ELL_TILE);
             load_w<WT, NOPS>(wp + (size_t)q * wstride, w[q]);
         } else {
             nb[q] = 0;
@@ -133,7 +160,7 @@ __device__ __forceinline__ void ell_gather_dot(const int32_t* __restrict__ idx_t
         for (int q = 0; q < UB; q++) {
             int j = j0 + UB + q;
             if (j < width) {
-                nb2[q] = __builtin_nontemporal_load(ip + (size_t)j * n_pad);
+                nb2[q] = __builtin_nontemporal_load(ip + (size_t)j * ELL_TILE);
                 load_w<WT, NOPS>(wp + (size_t)j * wstride, w2[q]);
             } else {
                 nb2[q] = 0;
@@ -161,6 +188,308 @@ __device__ __forceinline__ void ell_gather_dot(const int32_t* __restrict__ idx_t
     }
 }
 
+// ---- shared slot loads across the G lanes of one row (G = 2 or 4) ----------------------
+// When G lanes work on one row (G field groups), the plain core above has all G lanes load
+// the same index and weights for every slot. Here lane q of the group loads the index and
+// weights of slot j0 + q only (G slots per load instruction, no duplicates), and the values
+// are broadcast inside the group with DPP quad permutes (VALU moves, no LDS, no memory).
+// Fewer and narrower vector-memory instructions per slot: index loads drop by G, weight
+// loads by G in count and in bytes.
+template <int G, int S>
+__device__ __forceinline__ int bcast_group(int v) {
+    static_assert(G == 2 || G == 4, "DPP group broadcast handles groups of 2 or 4 lanes");
+    // quad_perm selectors: every lane of the group reads lane S of the group
+    constexpr int ctrl = (G == 4) ? (S * 0x55) : (S == 0 ? 0xA0 : 0xF5);
+    return __builtin_amdgcn_update_dpp(0, v, ctrl, 0xF, 0xF, false);
+}
+template <int G, int S>
+__device__ __forceinline__ float bcast_group(float v) {
+    return __int_as_float(bcast_group<G, S>(__float_as_int(v)));
+}
+template <int G, int S>
+__device__ __forceinline__ double bcast_group(double v) {
+    long long b = __double_as_longlong(v);
+    int lo = bcast_group<G, S>((int)(b & 0xffffffffll));
+    int hi = bcast_group<G, S>((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+template <int I, int N>
+struct StaticFor {
+    template <typename Fn>
+    __device__ __forceinline__ static void run(Fn&& fn) {
+        if constexpr (I < N) {
+            fn(std::integral_constant<int, I>{});
+            StaticFor<I + 1, N>::run(fn);
+        }
+    }
+};
+
+// Same result and summation order as ell_gather_dot; q = lane index inside the row's group.
+// Requires UB % G == 0 and the G lanes of a group to be active together (they share a row).
+template <typename WT, typename UT, typename AT, int NOPS, int FPT, int UB, int G>
+__device__ __forceinline__ void ell_gather_dot_shared(const int32_t* __restrict__ idx_t, const WT* __restrict__ w_t,
+                                                      const UT* __restrict__ u, int row, int width, int F,
+                                                      int fofs, int q, AT (&acc)[NOPS][FPT]) {
+    static_assert(UB % G == 0, "batch must be a multiple of the group size");
+    constexpr int L = UB / G;  // own loads per batch
+#pragma unroll
+    for (int o = 0; o < NOPS; o++)
+#pragma unroll
+        for (int f = 0; f < FPT; f++) acc[o][f] = (AT)0;
+    const size_t base = ell_row_base(row, width);
+    const int32_t* ip = idx_t + base;
+    const WT* wp = w_t + base * NOPS;
+    const size_t wstride = (size_t)ELL_TILE * NOPS;
+    const UT* ub = u + fofs;
+    int nbl[L];
+    WT wl[L][NOPS];
+#pragma unroll
+    for (int l = 0; l < L; l++) {
+        int j = l * G + q;
+        if (j < width) {
+            nbl[l] = __builtin_nontemporal_load(ip + (size_t)j * ELL_TILE);
+            load_w<WT, NOPS>(wp + (size_t)j * wstride, wl[l]);
+        } else {
+            nbl[l] = 0;
+#pragma unroll
+            for (int o = 0; o < NOPS; o++) wl[l][o] = (WT)0;
+        }
+    }
+    for (int j0 = 0; j0 < width; j0 += UB) {
+        // neighbour indices of the whole batch first: the gathers depend on them
+        int nb[UB];
+#pragma unroll
+        for (int l = 0; l < L; l++)
+            StaticFor<0, G>::run([&](auto S) { nb[l * G + decltype(S)::value] = bcast_group<G, decltype(S)::value>(nbl[l]); });
+        UT v[UB][FPT];
+#pragma unroll
+        for (int k = 0; k < UB; k++)
+            if (j0 + k < width) load_pack<UT, FPT>(ub + (size_t)((unsigned)nb[k] * (unsigned)F), v[k]);
+        // next batch's own loads go to a second small buffer; this batch's weights stay in
+        // wl and are broadcast only where the multiply-adds use them (few live registers)
+        int nbn[L];
+        WT wn[L][NOPS];
+#pragma unroll
+        for (int l = 0; l < L; l++) {
+            int j = j0 + UB + l * G + q;
+            if (j < width) {
+                nbn[l] = __builtin_nontemporal_load(ip + (size_t)j * ELL_TILE);
+                load_w<WT, NOPS>(wp + (size_t)j * wstride, wn[l]);
+            } else {
+                nbn[l] = 0;
+#pragma unroll
+                for (int o = 0; o < NOPS; o++) wn[l][o] = (WT)0;
+            }
+        }
+#pragma unroll
+        for (int l = 0; l < L; l++) {
+            StaticFor<0, G>::run([&](auto S) {
+                constexpr int s = decltype(S)::value;
+                const int k = l * G + s;
+                if (j0 + k < width) {
+#pragma unroll
+                    for (int o = 0; o < NOPS; o++) {
+                        AT a = (AT)bcast_group<G, s>(wl[l][o]);
+#pragma unroll
+                        for (int f = 0; f < FPT; f++) acc[o][f] = fma_acc<AT>(a, (AT)v[k][f], acc[o][f]);
+                    }
+                }
+            });
+        }
+#pragma unroll
+        for (int l = 0; l < L; l++) {
+            nbl[l] = nbn[l];
+#pragma unroll
+            for (int o = 0; o < NOPS; o++) wl[l][o] = wn[l][o];
+        }
+    }
+}
+
+// Shared loads with a deeper pipeline: the own index/weight loads run PD batches ahead (a
+// ring of PD register slots, unrolled so every slot index is a compile-time constant), and
+// the gathers of batch b+1 are issued before the multiply-adds of batch b wait for their
+// own gathers. More bytes in flight per wave for the same occupancy; same summation order.
+template <typename WT, int NOPS, int G, int L>
+__device__ __forceinline__ void own_loads(const int32_t* __restrict__ ip, const WT* __restrict__ wp,
+                                          size_t wstride, int width, int jb, int q, int (&nbl)[L],
+                                          WT (&wl)[L][NOPS]) {
+#pragma unroll
+    for (int l = 0; l < L; l++) {
+        int j = jb + l * G + q;
+        if (j < width) {
+            nbl[l] = __builtin_nontemporal_load(ip + (size_t)j * ELL_TILE);
+            load_w<WT, NOPS>(wp + (size_t)j * wstride, wl[l]);
+        } else {
+            nbl[l] = 0;
+#pragma unroll
+            for (int o = 0; o < NOPS; o++) wl[l][o] = (WT)0;
+        }
+    }
+}
+
+template <typename WT, typename UT, typename AT, int NOPS, int FPT, int UB, int G, int PD>
+__device__ __forceinline__ void ell_gather_dot_pipe(const int32_t* __restrict__ idx_t, const WT* __restrict__ w_t,
+                                                    const UT* __restrict__ u, int row, int width, int F,
+                                                    int fofs, int q, AT (&acc)[NOPS][FPT]) {
+    static_assert(UB % G == 0, "batch must be a multiple of the group size");
+    constexpr int L = UB / G;
+#pragma unroll
+    for (int o = 0; o < NOPS; o++)
+#pragma unroll
+        for (int f = 0; f < FPT; f++) acc[o][f] = (AT)0;
+    const size_t base = ell_row_base(row, width);
+    const int32_t* ip = idx_t + base;
+    const WT* wp = w_t + base * NOPS;
+    const size_t wstride = (size_t)ELL_TILE * NOPS;
+    const UT* ub = u + fofs;
+    const int nbatch = (width + UB - 1) / UB;
+    int nbl[PD][L];
+    WT wl[PD][L][NOPS];
+    StaticFor<0, PD>::run([&](auto S) {
+        constexpr int s = decltype(S)::value;
+        own_loads<WT, NOPS, G, L>(ip, wp, wstride, width, s * UB, q, nbl[s], wl[s]);
+    });
+    UT vc[UB][FPT];
+    auto gather = [&](const int (&nblr)[L], int jb, UT (&v)[UB][FPT]) {
+#pragma unroll
+        for (int l = 0; l < L; l++)
+            StaticFor<0, G>::run([&](auto S) {
+                constexpr int s = decltype(S)::value;
+                const int k = l * G + s;
+                const int nb = bcast_group<G, s>(nblr[l]);
+                if (jb + k < width) load_pack<UT, FPT>(ub + (size_t)((unsigned)nb * (unsigned)F), v[k]);
+            });
+    };
+    gather(nbl[0], 0, vc);
+    for (int b0 = 0; b0 < nbatch; b0 += PD) {
+        StaticFor<0, PD>::run([&](auto S) {
+            constexpr int s = decltype(S)::value;
+            constexpr int sn = (s + 1) % PD;
+            const int b = b0 + s;
+            if (b < nbatch) {
+                const int jb = b * UB;
+                UT vn[UB][FPT];
+                if (b + 1 < nbatch) gather(nbl[sn], jb + UB, vn);
+#pragma unroll
+                for (int l = 0; l < L; l++)
+                    StaticFor<0, G>::run([&](auto S2) {
+                        constexpr int s2 = decltype(S2)::value;
+                        const int k = l * G + s2;
+                        if (jb + k < width) {
+#pragma unroll
+                            for (int o = 0; o < NOPS; o++) {
+                                AT a = (AT)bcast_group<G, s2>(wl[s][l][o]);
+#pragma unroll
+                                for (int f = 0; f < FPT; f++) acc[o][f] = fma_acc<AT>(a, (AT)vc[k][f], acc[o][f]);
+                            }
+                        }
+                    });
+                own_loads<WT, NOPS, G, L>(ip, wp, wstride, width, jb + PD * UB, q, nbl[s], wl[s]);
+#pragma unroll
+                for (int k = 0; k < UB; k++)
+#pragma unroll
+                    for (int f = 0; f < FPT; f++) vc[k][f] = vn[k][f];
+            }
+        });
+    }
+}
+
+// ---- LDS-staged gathers ---------------------------------------------------------------
+// A block owns one tile of lt consecutive positions. It first copies the tile's distinct
+// neighbour records (F values each; F * sizeof(UT) a multiple of 16 bytes) from the field
+// array into LDS, 16 bytes per thread per step, so each record crosses the memory system
+// once per tile instead of once per (row, slot); the rows then gather from LDS through
+// 16-bit local indices.
+template <typename UT>
+__device__ __forceinline__ void ell_stage_tile(const UT* __restrict__ u, int F, const int32_t* __restrict__ nodes, int K,
+                                               UT* lds) {
+    typedef unsigned v4u __attribute__((ext_vector_type(4)));
+    const int C = F * (int)sizeof(UT) / 16;
+    for (int q = threadIdx.x; q < K * C; q += blockDim.x) {
+        const int k = q / C, c = q - k * C;
+        reinterpret_cast<v4u*>(lds)[q] = reinterpret_cast<const v4u*>(u + (size_t)nodes[k] * F)[c];
+    }
+}
+
+template <int G, int L, int NOPS, typename WT>
+__device__ __forceinline__ void own_loads_loc(const uint16_t* __restrict__ lip, const WT* __restrict__ wp,
+                                              size_t wstride, int width, int jb, int q, int (&nbl)[L],
+                                              WT (&wl)[L][NOPS]) {
+#pragma unroll
+    for (int l = 0; l < L; l++) {
+        int j = jb + l * G + q;
+        if (j < width) {
+            nbl[l] = (int)__builtin_nontemporal_load(lip + (size_t)j * ELL_TILE);
+            load_w<WT, NOPS>(wp + (size_t)j * wstride, wl[l]);
+        } else {
+            nbl[l] = 0;
+#pragma unroll
+            for (int o = 0; o < NOPS; o++) wl[l][o] = (WT)0;
+        }
+    }
+}
+
+// Same result and summation order as ell_gather_dot, reading the neighbour records from the
+// staged LDS copy: lane q of the row's G-lane group loads the local index and weights of
+// slot jb + q, PD batches ahead, and the values are broadcast in the group (DPP).
+template <typename WT, typename UT, typename AT, int NOPS, int FPT, int UB, int G, int PD>
+__device__ __forceinline__ void ell_gather_dot_lds(const uint16_t* __restrict__ loc_idx, const WT* __restrict__ w_t,
+                                                   const UT* lds, int p, int width, int F, int fofs, int q,
+                                                   AT (&acc)[NOPS][FPT]) {
+    static_assert(UB % G == 0, "batch must be a multiple of the group size");
+    constexpr int L = UB / G;
+#pragma unroll
+    for (int o = 0; o < NOPS; o++)
+#pragma unroll
+        for (int f = 0; f < FPT; f++) acc[o][f] = (AT)0;
+    const size_t base = ell_row_base(p, width);
+    const uint16_t* lip = loc_idx + base;
+    const WT* wp = w_t + base * NOPS;
+    const size_t wstride = (size_t)ELL_TILE * NOPS;
+    const UT* lb = lds + fofs;
+    const int nbatch = (width + UB - 1) / UB;
+    int nbl[PD][L];
+    WT wl[PD][L][NOPS];
+    StaticFor<0, PD>::run([&](auto S) {
+        constexpr int s = decltype(S)::value;
+        own_loads_loc<G, L, NOPS, WT>(lip, wp, wstride, width, s * UB, q, nbl[s], wl[s]);
+    });
+    for (int b0 = 0; b0 < nbatch; b0 += PD) {
+        StaticFor<0, PD>::run([&](auto S) {
+            constexpr int s = decltype(S)::value;
+            const int b = b0 + s;
+            if (b < nbatch) {
+                const int jb = b * UB;
+                UT v[UB][FPT];
+#pragma unroll
+                for (int l = 0; l < L; l++)
+                    StaticFor<0, G>::run([&](auto S2) {
+                        constexpr int s2 = decltype(S2)::value;
+                        const int k = l * G + s2;
+                        const int li = bcast_group<G, s2>(nbl[s][l]);
+                        if (jb + k < width) load_pack<UT, FPT>(lb + li * F, v[k]);
+                    });
+#pragma unroll
+                for (int l = 0; l < L; l++)
+                    StaticFor<0, G>::run([&](auto S2) {
+                        constexpr int s2 = decltype(S2)::value;
+                        const int k = l * G + s2;
+                        if (jb + k < width) {
+#pragma unroll
+                            for (int o = 0; o < NOPS; o++) {
+                                AT a = (AT)bcast_group<G, s2>(wl[s][l][o]);
+#pragma unroll
+                                for (int f = 0; f < FPT; f++) acc[o][f] = fma_acc<AT>(a, (AT)v[k][f], acc[o][f]);
+                            }
+                        }
+                    });
+                own_loads_loc<G, L, NOPS, WT>(lip, wp, wstride, width, jb + PD * UB, q, nbl[s], wl[s]);
+            }
+        });
+    }
+}
+
 // Maps a hardware block index to a logical tile so that each of the 8 XCDs (which receive
 // blocks round-robin) works on one contiguous eighth of the rows and keeps its gathered
 // neighbourhood in its own L2.
@@ -173,7 +502,8 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
 struct EllTune {
     int fpt = 1;
     int block = 256;
-    int ub = 4;  // slots per gather batch (4 or 8)
+    int ub = 4;     // slots per gather batch (4 or 8)
+    int share = 0;  // 1: the G = F/fpt lanes of a row split the index/weight loads (G = 2 or 4)
 };
 
 int rbfx_ell_pick(rbfx_ell* ell, int n_fields, int field_type, int accum, int kind, EllTune* t);

@@ -23,6 +23,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <iterator>
 #include <unordered_map>
 #include <vector>
 #ifdef _OPENMP
@@ -266,6 +267,65 @@ inline double unit_sym(uint64_t r) { return (double)(r >> 11) * (2.0 / 900719925
 
 }  // namespace
 
+int rbfx_build_local_tables(const int32_t* idx_layout, int n_rows, int n_pad, int width, int tile_layout, int lt,
+                            int cap, std::vector<int32_t>& pos, std::vector<int32_t>& off, std::vector<int32_t>& nodes,
+                            std::vector<uint16_t>& loc_idx) {
+    auto entry = [&](int p, int j) {  // layout offset of (position p, slot j)
+        return ((size_t)(p / tile_layout) * (size_t)width + (size_t)j) * tile_layout + (size_t)(p % tile_layout);
+    };
+    // greedy tiles inside independent chunks of positions (chunks run in parallel)
+    const int chunk = 64 * lt;
+    const int n_chunks = (n_rows + chunk - 1) / chunk;
+    std::vector<std::vector<int32_t>> c_pos((size_t)n_chunks);                 // tile starts per chunk
+    std::vector<std::vector<std::vector<int32_t>>> c_nodes((size_t)n_chunks);  // node list per tile
+    loc_idx.assign((size_t)n_pad * width, 0);
+    int too_big = 0;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : too_big)
+#endif
+    for (int c = 0; c < n_chunks; c++) {
+        const int p_lo = c * chunk, p_hi = std::min(n_rows, (c + 1) * chunk);
+        std::vector<int32_t> u, merged, row((size_t)width);
+        int p = p_lo;
+        while (p < p_hi) {
+            const int start = p;
+            u.clear();
+            while (p < p_hi && p - start < lt) {
+                for (int j = 0; j < width; j++) row[(size_t)j] = idx_layout[entry(p, j)];
+                std::sort(row.begin(), row.end());
+                merged.clear();
+                std::set_union(u.begin(), u.end(), row.begin(), row.end(), std::back_inserter(merged));
+                if (cap > 0 && p > start && (int)merged.size() > cap) break;
+                u.swap(merged);
+                p++;
+            }
+            if (u.size() > 65535) too_big++;
+            for (int q = start; q < p; q++)
+                for (int j = 0; j < width; j++) {
+                    size_t e = entry(q, j);
+                    loc_idx[e] = (uint16_t)(std::lower_bound(u.begin(), u.end(), idx_layout[e]) - u.begin());
+                }
+            c_pos[(size_t)c].push_back(start);
+            c_nodes[(size_t)c].push_back(u);
+        }
+    }
+    if (too_big) return -1;
+    pos.clear();
+    off.assign(1, 0);
+    nodes.clear();
+    int k_max = 0;
+    for (int c = 0; c < n_chunks; c++)
+        for (size_t t = 0; t < c_pos[(size_t)c].size(); t++) {
+            const std::vector<int32_t>& u = c_nodes[(size_t)c][t];
+            pos.push_back(c_pos[(size_t)c][t]);
+            nodes.insert(nodes.end(), u.begin(), u.end());
+            off.push_back((int32_t)nodes.size());
+            k_max = std::max(k_max, (int)u.size());
+        }
+    pos.push_back(n_rows);
+    return k_max;
+}
+
 extern "C" {
 
 int rbfx_knn_kdtree(int32_t n_nodes, const double* xyz, int32_t k, int32_t* out_idx, int32_t n_threads) {
@@ -333,6 +393,47 @@ int rbfx_nodes_icosahedral(int32_t level, double* xyz, int32_t* out_n) {
     }
     if ((int64_t)(v.size() / 3) != n_total) return RBFX_E_INVALID;
     std::memcpy(xyz, v.data(), v.size() * sizeof(double));
+    return RBFX_OK;
+}
+
+// Processing order along a 3-D Morton (Z-order) curve over the nodes' bounding box, 21 bits
+// per axis: out_order[p] = node visited p-th, ties broken by node index. Nodes that are close
+// in space end up close in the order, so rows processed together share stencil neighbours.
+int rbfx_order_morton(int32_t n_nodes, const double* xyz, int32_t* out_order) {
+    if (n_nodes <= 0 || !xyz || !out_order) return RBFX_E_INVALID;
+    double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+    for (int32_t i = 0; i < n_nodes; i++)
+        for (int d = 0; d < 3; d++) {
+            double c = xyz[3 * (size_t)i + d];
+            if (!std::isfinite(c)) return RBFX_E_INVALID;
+            lo[d] = std::min(lo[d], c);
+            hi[d] = std::max(hi[d], c);
+        }
+    auto spread = [](uint64_t v) {  // 21 bits -> every third bit of 63
+        v &= 0x1fffffull;
+        v = (v | v << 32) & 0x1f00000000ffffull;
+        v = (v | v << 16) & 0x1f0000ff0000ffull;
+        v = (v | v << 8) & 0x100f00f00f00f00full;
+        v = (v | v << 4) & 0x10c30c30c30c30c3ull;
+        v = (v | v << 2) & 0x1249249249249249ull;
+        return v;
+    };
+    std::vector<std::pair<uint64_t, int32_t>> keys((size_t)n_nodes);
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+    for (int32_t i = 0; i < n_nodes; i++) {
+        uint64_t k = 0;
+        for (int d = 0; d < 3; d++) {
+            double span = hi[d] - lo[d];
+            double t = span > 0 ? (xyz[3 * (size_t)i + d] - lo[d]) / span : 0.0;
+            uint64_t q = (uint64_t)std::min(2097151.0, std::max(0.0, std::floor(t * 2097152.0)));
+            k |= spread(q) << d;
+        }
+        keys[(size_t)i] = {k, i};
+    }
+    std::sort(keys.begin(), keys.end());
+    for (int32_t p = 0; p < n_nodes; p++) out_order[p] = keys[(size_t)p].second;
     return RBFX_OK;
 }
 

@@ -34,6 +34,7 @@ struct rbfx_rk4 {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     double graph_dt = 0.0;
+    int graph_layout = -1;  // operator-set layout version the graph was captured with
 };
 
 namespace {
@@ -41,7 +42,8 @@ namespace {
 struct StageArgs {
     const int32_t* idx_t;
     const void* w_t;
-    int n_rows, n_pad, width, F;
+    const int32_t* order;
+    int n_rows, width, F;
     const void* xin;
     double* u;
     double* acc;
@@ -50,9 +52,19 @@ struct StageArgs {
     double hdt, dt, dt6;
     int stage;
     int n_tiles, nb;
+    // LDS staging of the operator set (rk4_stage_lds_kernel only)
+    const int32_t* loc_pos;
+    const int32_t* loc_off;
+    const int32_t* loc_nodes;
+    const uint16_t* loc_idx;
+    int lt;
 };
 
-template <typename WT, typename UT, typename AT, int NOPS, int FPT, int UB>
+// Right-hand side f = sum_o c_o (D_o X) and the stage update of one thread's FPT fields.
+template <typename UT, typename AT, int NOPS, int FPT>
+__device__ __forceinline__ void stage_update(const StageArgs& a, int row, int fofs, const AT (&d)[NOPS][FPT]);
+
+template <typename WT, typename UT, typename AT, int NOPS, int FPT, int UB, int SG, int PD>
 __global__ void __launch_bounds__(256) rk4_stage_kernel(StageArgs a) {
     const int G = a.F / FPT;
     const int R = blockDim.x / G;
@@ -61,11 +73,48 @@ __global__ void __launch_bounds__(256) rk4_stage_kernel(StageArgs a) {
     int t = threadIdx.x;
     int rl = t / G;
     if (rl >= R) return;
-    int row = tile * R + rl;
-    if (row >= a.n_rows) return;
+    const int p = tile * R + rl;  // processing position (operator layout order)
+    if (p >= a.n_rows) return;
+    const int row = ell_row_at(a.order, p);  // node whose state this thread advances
     int fofs = (t - rl * G) * FPT;
     AT d[NOPS][FPT];
-    ell_gather_dot<WT, UT, AT, NOPS, FPT, UB>(a.idx_t, (const WT*)a.w_t, (const UT*)a.xin, row, a.n_pad, a.width, a.F, fofs, d);
+    if constexpr (SG == 0)
+        ell_gather_dot<WT, UT, AT, NOPS, FPT, UB>(a.idx_t, (const WT*)a.w_t, (const UT*)a.xin, p, a.width, a.F, fofs,
+                                                  d);
+    else if constexpr (PD == 0)  // the launcher guarantees G == SG
+        ell_gather_dot_shared<WT, UT, AT, NOPS, FPT, UB, SG>(a.idx_t, (const WT*)a.w_t, (const UT*)a.xin, p, a.width,
+                                                             a.F, fofs, t - rl * G, d);
+    else
+        ell_gather_dot_pipe<WT, UT, AT, NOPS, FPT, UB, SG, PD>(a.idx_t, (const WT*)a.w_t, (const UT*)a.xin, p,
+                                                               a.width, a.F, fofs, t - rl * G, d);
+    stage_update<UT, AT, NOPS, FPT>(a, row, fofs, d);
+}
+
+// LDS-staged variant: block = one tile of a.lt positions (a.lt * G threads); the tile's
+// distinct neighbour records are copied into LDS first, then every row gathers from there.
+template <typename WT, typename UT, typename AT, int NOPS, int FPT, int UB, int SG, int PD>
+__global__ void __launch_bounds__(256) rk4_stage_lds_kernel(StageArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    UT* lds = reinterpret_cast<UT*>(smem);
+    const int tile = xcd_tile(blockIdx.x, a.nb);
+    if (tile >= a.n_tiles) return;  // uniform over the block
+    const int t = threadIdx.x;
+    const int k0 = a.loc_off[tile];
+    ell_stage_tile<UT>((const UT*)a.xin, a.F, a.loc_nodes + k0, a.loc_off[tile + 1] - k0, lds);
+    __syncthreads();
+    const int rl = t / SG;
+    const int p = a.loc_pos[tile] + rl;
+    if (p >= a.loc_pos[tile + 1]) return;  // after the only barrier; tiles hold <= lt rows
+    const int row = ell_row_at(a.order, p);
+    const int fofs = (t - rl * SG) * FPT;
+    AT d[NOPS][FPT];
+    ell_gather_dot_lds<WT, UT, AT, NOPS, FPT, UB, SG, PD>(a.loc_idx, (const WT*)a.w_t, lds, p, a.width, a.F, fofs,
+                                                          t - rl * SG, d);
+    stage_update<UT, AT, NOPS, FPT>(a, row, fofs, d);
+}
+
+template <typename UT, typename AT, int NOPS, int FPT>
+__device__ __forceinline__ void stage_update(const StageArgs& a, int row, int fofs, const AT (&d)[NOPS][FPT]) {
     double c[NOPS];
 #pragma unroll
     for (int o = 0; o < NOPS; o++) c[o] = a.coef[o] ? a.coef[o][row] : 1.0;
@@ -123,16 +172,44 @@ int launch_stage_fpt(rbfx_rk4* rk, StageArgs& a, int block) {
     hipStream_t s = rk->ctx->stream;
     int nb = a.nb;
     const int fpt = rk->tune.fpt, ub = rk->tune.ub;
-#define RBFX_STAGE_CASE(FPT_, UB_)                                           \
-    if (fpt == FPT_ && ub == UB_) {                                          \
-        rk4_stage_kernel<WT, UT, AT, NOPS, FPT_, UB_><<<nb, block, 0, s>>>(a); \
-        RBFX_LAUNCH_CHECK(rk->ctx, "rk4_stage_kernel");                     \
-        return RBFX_OK;                                                      \
+    const int sg = rk->tune.share ? a.F / fpt : 0;  // validated by the tuner: 2 or 4, dividing ub
+    if (rk->tune.share == 4) {
+        const size_t lds = (size_t)rk->ell->k_max * a.F * sizeof(UT);
+#define RBFX_LDS_CASE(FPT_, UB_, SG_)                                                                   \
+    if (fpt == FPT_ && ub == UB_ && sg == SG_) {                                                        \
+        rk4_stage_lds_kernel<WT, UT, AT, NOPS, FPT_, UB_, SG_, 2><<<nb, block, lds, s>>>(a);            \
+        RBFX_LAUNCH_CHECK(rk->ctx, "rk4_stage_lds_kernel");                                            \
+        return RBFX_OK;                                                                                 \
     }
-    RBFX_STAGE_CASE(1, 4) RBFX_STAGE_CASE(2, 4) RBFX_STAGE_CASE(4, 4) RBFX_STAGE_CASE(8, 4)
-    RBFX_STAGE_CASE(1, 8) RBFX_STAGE_CASE(2, 8) RBFX_STAGE_CASE(4, 8) RBFX_STAGE_CASE(8, 8)
+        if constexpr (NOPS >= 3) {
+            RBFX_LDS_CASE(4, 8, 4) RBFX_LDS_CASE(4, 4, 4) RBFX_LDS_CASE(8, 8, 2) RBFX_LDS_CASE(8, 4, 2)
+        }
+#undef RBFX_LDS_CASE
+        return rbfx_fail(rk->ctx, RBFX_E_INVALID, "rk4: no LDS-staged kernel for fields/thread %d, batch %d, group %d",
+                         fpt, ub, sg);
+    }
+    const int pd = rk->tune.share >= 2 ? rk->tune.share : 0;  // prefetch depth of the pipelined core
+#define RBFX_STAGE_CASE(FPT_, UB_, SG_, PD_)                                               \
+    if (fpt == FPT_ && ub == UB_ && sg == SG_ && pd == PD_) {                              \
+        rk4_stage_kernel<WT, UT, AT, NOPS, FPT_, UB_, SG_, PD_><<<nb, block, 0, s>>>(a); \
+        RBFX_LAUNCH_CHECK(rk->ctx, "rk4_stage_kernel");                                   \
+        return RBFX_OK;                                                                    \
+    }
+    RBFX_STAGE_CASE(1, 4, 0, 0) RBFX_STAGE_CASE(2, 4, 0, 0) RBFX_STAGE_CASE(4, 4, 0, 0) RBFX_STAGE_CASE(8, 4, 0, 0)
+    RBFX_STAGE_CASE(1, 8, 0, 0) RBFX_STAGE_CASE(2, 8, 0, 0) RBFX_STAGE_CASE(4, 8, 0, 0) RBFX_STAGE_CASE(8, 8, 0, 0)
+    RBFX_STAGE_CASE(2, 4, 2, 0) RBFX_STAGE_CASE(4, 4, 2, 0) RBFX_STAGE_CASE(8, 4, 2, 0)
+    RBFX_STAGE_CASE(2, 8, 2, 0) RBFX_STAGE_CASE(4, 8, 2, 0) RBFX_STAGE_CASE(8, 8, 2, 0)
+    RBFX_STAGE_CASE(2, 4, 4, 0) RBFX_STAGE_CASE(4, 4, 4, 0) RBFX_STAGE_CASE(8, 4, 4, 0)
+    RBFX_STAGE_CASE(2, 8, 4, 0) RBFX_STAGE_CASE(4, 8, 4, 0) RBFX_STAGE_CASE(8, 8, 4, 0)
+    if constexpr (NOPS >= 3) {  // pipelined variants for the multi-operator sets
+        RBFX_STAGE_CASE(4, 4, 4, 2) RBFX_STAGE_CASE(4, 8, 4, 2) RBFX_STAGE_CASE(4, 4, 4, 3) RBFX_STAGE_CASE(4, 8, 4, 3)
+        RBFX_STAGE_CASE(8, 4, 2, 2) RBFX_STAGE_CASE(8, 8, 2, 2) RBFX_STAGE_CASE(8, 4, 2, 3) RBFX_STAGE_CASE(8, 8, 2, 3)
+        RBFX_STAGE_CASE(4, 4, 2, 2) RBFX_STAGE_CASE(4, 8, 2, 2) RBFX_STAGE_CASE(2, 4, 4, 2) RBFX_STAGE_CASE(2, 8, 4, 2)
+    }
 #undef RBFX_STAGE_CASE
-    return rbfx_fail(rk->ctx, RBFX_E_INVALID, "rk4: unsupported shape (fields/thread %d, batch %d)", fpt, ub);
+    return rbfx_fail(rk->ctx, RBFX_E_INVALID,
+                     "rk4: unsupported shape (fields/thread %d, batch %d, shared group %d, prefetch %d)", fpt, ub, sg,
+                     pd);
 }
 
 template <typename WT, typename UT, typename AT>
@@ -152,8 +229,8 @@ int launch_stage(rbfx_rk4* rk, int stage, double dt) {
     std::memset(&a, 0, sizeof(a));
     a.idx_t = e->idx_t;
     a.w_t = e->w_t;
+    a.order = e->order;
     a.n_rows = e->n_rows;
-    a.n_pad = e->n_pad;
     a.width = e->width;
     a.F = rk->F;
     for (int o = 0; o < 4; o++) a.coef[o] = rk->coef[o];
@@ -173,8 +250,17 @@ int launch_stage(rbfx_rk4* rk, int stage, double dt) {
     }
     int block = rk->tune.block;
     int G = rk->F / rk->tune.fpt;
+    if (rk->tune.share == 4) {  // LDS-staged: one block per tile of lt positions
+        if (e->lt <= 0) return rbfx_fail(rk->ctx, RBFX_E_STATE, "rk4: LDS-staged shape without rbfx_ell_build_local");
+        block = e->lt * G;
+        a.loc_pos = e->loc_pos;
+        a.loc_off = e->loc_off;
+        a.loc_nodes = e->loc_nodes;
+        a.loc_idx = e->loc_idx;
+        a.lt = e->lt;
+    }
     int R = block / G;
-    a.n_tiles = (e->n_rows + R - 1) / R;
+    a.n_tiles = rk->tune.share == 4 ? e->n_ltiles : (e->n_rows + R - 1) / R;
     a.nb = (a.n_tiles + 7) / 8 * 8;
     if (e->storage == RBFX_F64) return launch_stage_ops<double, double, double>(rk, a, block);
     if (rk->accum == RBFX_F64) return launch_stage_ops<float, float, double>(rk, a, block);
@@ -195,6 +281,7 @@ int capture_step(rbfx_rk4* rk, double dt) {
     rk->graph = g;
     RBFX_HIP_CHECK(ctx, hipGraphInstantiate(&rk->exec, rk->graph, nullptr, nullptr, 0));
     rk->graph_dt = dt;
+    rk->graph_layout = rk->ell->layout_version;
     return RBFX_OK;
 }
 
@@ -280,10 +367,12 @@ int rbfx_rk4_step(rbfx_rk4* rk, double dt, int32_t n_steps) {
     EllTune t;
     int prc = rbfx_ell_pick(rk->ell, rk->F, rk->field_storage, rk->accum, 1, &t);
     if (prc) return prc;
-    bool shape_changed = (t.fpt != rk->tune.fpt || t.block != rk->tune.block || t.ub != rk->tune.ub);
+    bool shape_changed = (t.fpt != rk->tune.fpt || t.block != rk->tune.block || t.ub != rk->tune.ub ||
+                          t.share != rk->tune.share);
     rk->tune = t;
     if (rk->use_graph) {
-        if (!rk->exec || rk->graph_dt != dt || shape_changed) {
+        // the captured launches hold the operator set's buffers: recapture after a re-layout
+        if (!rk->exec || rk->graph_dt != dt || shape_changed || rk->graph_layout != rk->ell->layout_version) {
             int rc = capture_step(rk, dt);
             if (rc) return rc;
         }

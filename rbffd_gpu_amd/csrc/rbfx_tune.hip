@@ -60,25 +60,53 @@ extern "C" {
 
 // Sets the launch shape used for (ell, n_fields, types, kind); kind 0 = SpMM, 1 = RK4 stage.
 int rbfx_tune_set(rbfx_ell* e, int32_t kind, int32_t n_fields, int32_t field_type, int32_t accum, int32_t fpt, int32_t block,
-                  int32_t batch) {
+                  int32_t batch, int32_t share) {
     if (!e) return RBFX_E_INVALID;
-    if (!valid(n_fields, fpt, block) || (batch != 4 && batch != 8))
-        return rbfx_fail(e->ctx, RBFX_E_INVALID, "tune_set: invalid shape fpt %d block %d batch %d for %d fields", fpt, block,
-                         batch, n_fields);
+    if (share == 4) {  // LDS-staged gathers (rbfx_ell_build_local); the block is the tile
+        const int G = fpt > 0 ? n_fields / fpt : 0;
+        const size_t rec = (size_t)n_fields * (field_type == RBFX_F64 ? 8 : 4);
+        const bool ok = kind == 1 && e->lt > 0 && e->n_ops >= 3 && fpt > 0 && n_fields % fpt == 0 &&
+                        ((fpt == 4 && G == 4) || (fpt == 8 && G == 2)) && (batch == 4 || batch == 8) &&
+                        e->lt * G <= 256 && rec % 16 == 0 && (size_t)e->k_max * rec <= (size_t)e->lds_limit;
+        if (!ok)
+            return rbfx_fail(e->ctx, RBFX_E_INVALID,
+                             "tune_set: no LDS-staged shape for %d fields, fpt %d, batch %d (tile %d, %d nodes per tile)",
+                             n_fields, fpt, batch, e->lt, e->k_max);
+        std::lock_guard<std::mutex> lk(g_mu);
+        g_table[make_key(e, n_fields, field_type, accum, kind)] = EllTune{fpt, e->lt * G, batch, share};
+        return RBFX_OK;
+    }
+    if (!valid(n_fields, fpt, block) || (batch != 4 && batch != 8) || share < 0 || share > 3)
+        return rbfx_fail(e->ctx, RBFX_E_INVALID, "tune_set: invalid shape fpt %d block %d batch %d share %d for %d fields",
+                         fpt, block, batch, share, n_fields);
+    if (share) {
+        int G = n_fields / fpt;
+        if (kind != 1 || (G != 2 && G != 4) || fpt == 1 || batch % G)
+            return rbfx_fail(e->ctx, RBFX_E_INVALID,
+                             "tune_set: shared loads need the RK4 stage kernel, 2 or 4 lanes per row, fpt > 1 and a "
+                             "batch divisible by them (fields %d, fpt %d, batch %d)", n_fields, fpt, batch);
+        // pipelined cores (share 2, 3 = prefetch depth) exist for 3-4 operators and these shapes
+        const bool pipe_ok = e->n_ops >= 3 && ((fpt == 4 && G == 4) || (fpt == 8 && G == 2) ||
+                                               (share == 2 && ((fpt == 4 && G == 2) || (fpt == 2 && G == 4))));
+        if (share >= 2 && !pipe_ok)
+            return rbfx_fail(e->ctx, RBFX_E_INVALID, "tune_set: no pipelined core for %d operators, fpt %d, %d lanes, depth %d",
+                             e->n_ops, fpt, G, share);
+    }
     std::lock_guard<std::mutex> lk(g_mu);
-    g_table[make_key(e, n_fields, field_type, accum, kind)] = EllTune{fpt, block, batch};
+    g_table[make_key(e, n_fields, field_type, accum, kind)] = EllTune{fpt, block, batch, share};
     return RBFX_OK;
 }
 
 int rbfx_tune_get(rbfx_ell* e, int32_t kind, int32_t n_fields, int32_t field_type, int32_t accum, int32_t* fpt, int32_t* block,
-                  int32_t* batch) {
-    if (!e || !fpt || !block || !batch) return RBFX_E_INVALID;
+                  int32_t* batch, int32_t* share) {
+    if (!e || !fpt || !block || !batch || !share) return RBFX_E_INVALID;
     EllTune t;
     int rc = rbfx_ell_pick(e, n_fields, field_type, accum, kind, &t);
     if (rc) return rc;
     *fpt = t.fpt;
     *block = t.block;
     *batch = t.ub;
+    *share = t.share;
     return RBFX_OK;
 }
 

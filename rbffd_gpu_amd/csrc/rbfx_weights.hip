@@ -6,10 +6,11 @@
 // and calls arma::solve (LAPACK dgesv: LU with partial pivoting). HV columns are scaled by
 // getHVScalar (295-297). Gaussian phi and its derivatives follow rbf_gaussian.h:40-249.
 //
-// GPU design: one workgroup per stencil. The node coordinates of the stencil are staged in
-// LDS, the augmented matrix [M | R] is assembled straight into LDS, factored in place by
-// right-looking elimination with implicit partial pivoting (pivot = largest |a_rk| among the
-// rows not yet used; the row order is kept in a permutation instead of moving rows), and
+// GPU design: one 64-lane wave per stencil, no block barriers. The node coordinates of the
+// stencil are staged in LDS, the augmented matrix [M | R] is assembled straight into LDS
+// (one row per lane), factored in place by right-looking elimination with implicit partial
+// pivoting (pivot = largest |a_rk| among the rows not yet used, found by one wave max over
+// packed keys; the row order is kept in a permutation instead of moving rows), and
 // back-substituted. Stencils are grouped by system size so each launch sizes LDS for its
 // class. FP32 mode factors a float copy and refines x with FP64 residuals recomputed from the
 // node coordinates; stencils whose refinement does not converge are re-solved in FP64.
@@ -22,7 +23,7 @@
 
 namespace {
 
-constexpr int WBLOCK = 256;
+
 constexpr int MAX_M = 128;
 constexpr int MAX_T = 16;
 constexpr size_t STATIC_LDS_RESERVE = 1024;  // static __shared__ scratch of the kernels (< 1 KB)
@@ -198,18 +199,28 @@ __device__ double mono_rhs(uint32_t type, int p, int sph, const double* x0, int 
     }
 }
 
-// ---- shared-memory layout of one system ----------------------------------------------------
-// [X: n*3 f64][SPH: n*2 f64][XS: m*T f64][BS: m*T f64][perm, pos: m i32 each][red: 8 f64 + 8 i32]
-// [Aug: m*ld of TA]; Aug = [M | R] with ld = m + T (+1 when even, so column walks spread over
-// the LDS banks). XS/BS (solution by unknown, original RHS by row) exist only in FP32 mode.
+// ---- one wave per system -----------------------------------------------------------------
+// Each system is solved by one 64-lane workgroup with no block barriers: lane L owns rows L
+// and L + 64 (m <= 128) of the augmented matrix [M | R] in LDS, so a rank-1 update is one
+// multiply-add per owned row and column, the pivot row is read by every lane at the same
+// address (an LDS broadcast), and all LDS traffic of a wave is ordered by the hardware.
+// Several systems share a CU through the LDS they need (about 33 KB at m = 59 in FP64).
+//
+// LDS layout of one system:
+// [X: n*3 f64][SPH: n*2 f64][XS: m*T f64][BS: m*T f64][perm: m i32][Aug: m*ld of TA]
+// Aug = [M | R] with ld = m + T (+1 when even: rows start on alternating banks, so the 64
+// lanes reading one column hit distinct banks). XS/BS (solution by unknown, original RHS by
+// row) exist only in FP32 mode.
+constexpr int WAVE = 64;
+constexpr int UCH = 8;  // columns per chunk of the rank-1 update (loads batched ahead of stores)
+
 __host__ __device__ inline int aug_ld(int m, int T) {
     int ld = m + T;
     return (ld & 1) ? ld : ld + 1;
 }
 
 __host__ __device__ inline size_t lds_head(int m, int n, int T, bool ir) {
-    size_t b = (size_t)n * 5 * sizeof(double) + (ir ? 2 * (size_t)m * T * sizeof(double) : 0) +
-               2 * (size_t)m * sizeof(int) + 8 * sizeof(double) + 8 * sizeof(int);
+    size_t b = (size_t)n * 5 * sizeof(double) + (ir ? 2 * (size_t)m * T * sizeof(double) : 0) + (size_t)m * sizeof(int);
     return (b + 15) / 16 * 16;
 }
 
@@ -238,9 +249,6 @@ struct Sys {
     double* XS;
     double* BS;
     int* perm;
-    int* pos;
-    double* red_v;
-    int* red_i;
 };
 
 __device__ __forceinline__ double rhs_entry(const Sys& S, int r, int t) {
@@ -252,162 +260,179 @@ __device__ __forceinline__ double rhs_entry(const Sys& S, int r, int t) {
     return mono_rhs(P.types[t], r - S.n, P.sph, S.X, P.dim, P.hv_k, lam0, th0);
 }
 
-// argmax over the block (ties -> smaller row); result in *out_p, visible after the caller's barrier.
-__device__ __forceinline__ void block_argmax(double v, int r, double* red_v, int* red_i, int* out_p) {
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v) {
+#pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
-        double ov = __shfl_xor(v, off, 64);
-        int oi = __shfl_xor(r, off, 64);
-        if (ov > v || (ov == v && oi < r)) { v = ov; r = oi; }
+        unsigned lo = __shfl_xor((unsigned)(v & 0xffffffffull), off, WAVE);
+        unsigned hi = __shfl_xor((unsigned)(v >> 32), off, WAVE);
+        unsigned long long o = ((unsigned long long)hi << 32) | lo;
+        v = o > v ? o : v;
     }
-    int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    if (lane == 0) { red_v[wv] = v; red_i[wv] = r; }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double bv = red_v[0];
-        int bi = red_i[0];
-        for (int q = 1; q < (int)(blockDim.x >> 6); q++)
-            if (red_v[q] > bv || (red_v[q] == bv && red_i[q] < bi)) { bv = red_v[q]; bi = red_i[q]; }
-        *out_p = bi;
-    }
+    return v;
 }
 
-// Right-looking elimination of [M | R] with implicit partial pivoting. Row perm[k] is the
-// pivot row of step k (pos[] is the inverse); the multiplier of row r at step k is kept in
-// Aug[r][k] so later right-hand sides can be eliminated. Returns false on an exact zero pivot.
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, WAVE));
+    return v;
+}
+
+// Pivot key of a candidate row: |a| with its lowest 8 mantissa bits replaced by 255 - r, so
+// one unsigned 64-bit max picks the largest |a| (to within 2^-44 relative) and, among equal
+// keys, the smaller row. Rows already used as pivots do not take part (key 0).
+__device__ __forceinline__ unsigned long long pivot_key(double a, int r) {
+    unsigned long long b = (unsigned long long)__double_as_longlong(fabs(a));
+    return (b & ~0xffull) | (unsigned long long)(255 - r);
+}
+
+// Right-looking elimination of [M | R] with implicit partial pivoting. perm[k] is the pivot
+// row of step k; posr[h] is the step at which this lane's row h was used as pivot (m = not
+// yet). The multiplier of row r at step k is kept in Aug[r][k] so that later right-hand
+// sides can be eliminated. Returns false on an exact zero pivot (uniform over the wave).
 template <typename TA>
-__device__ bool factor(Sys& S, TA* Aug, int* s_p) {
-    const int tid = threadIdx.x, nthr = blockDim.x;
-    const int lane = tid & 63, wv = tid >> 6, nw = nthr >> 6;
-    const int m = S.m, ld = S.ld, wcols = m + S.T;
-    for (int r = tid; r < m; r += nthr) S.pos[r] = m;  // m = not yet pivoted
-    __syncthreads();
+__device__ bool factor(const Sys& S, TA* Aug, int (&posr)[2]) {
+    const int lane = threadIdx.x, m = S.m, ld = S.ld, wcols = m + S.T;
+    const int r0 = lane, r1 = lane + WAVE;
+    posr[0] = posr[1] = m;
     for (int k = 0; k < m; k++) {
-        double v = -1.0;
-        int r0 = 0x7fffffff;
-        for (int r = tid; r < m; r += nthr) {
-            if (S.pos[r] == m) {
-                double a = fabs((double)Aug[(size_t)r * ld + k]);
-                if (a > v || (a == v && r < r0)) { v = a; r0 = r; }
+        unsigned long long key = 0;
+        if (r0 < m && posr[0] == m) key = pivot_key((double)Aug[r0 * ld + k], r0);
+        if (r1 < m && posr[1] == m) {
+            unsigned long long k1 = pivot_key((double)Aug[r1 * ld + k], r1);
+            key = k1 > key ? k1 : key;
+        }
+        key = wave_max_u64(key);
+        const int p = 255 - (int)(key & 0xffull);
+        const TA piv = Aug[p * ld + k];
+        if (piv == (TA)0) return false;
+        if (lane == 0) S.perm[k] = p;
+        if (r0 == p) posr[0] = k;
+        if (r1 == p) posr[1] = k;
+        const bool a0 = r0 < m && posr[0] == m, a1 = r1 < m && posr[1] == m;
+        const TA inv = (TA)1 / piv;
+        TA l0 = 0, l1 = 0;
+        if (a0) { l0 = Aug[r0 * ld + k] * inv; Aug[r0 * ld + k] = l0; }
+        if (a1) { l1 = Aug[r1 * ld + k] * inv; Aug[r1 * ld + k] = l1; }
+        const TA* prow = Aug + p * ld;
+        TA* row0 = Aug + r0 * ld;
+        TA* row1 = Aug + r1 * ld;
+        int c = k + 1;
+        for (; c + UCH <= wcols; c += UCH) {
+            TA pv[UCH], v0[UCH], v1[UCH];
+#pragma unroll
+            for (int q = 0; q < UCH; q++) pv[q] = prow[c + q];
+            if (a0) {
+#pragma unroll
+                for (int q = 0; q < UCH; q++) v0[q] = row0[c + q];
+            }
+            if (a1) {
+#pragma unroll
+                for (int q = 0; q < UCH; q++) v1[q] = row1[c + q];
+            }
+            if (a0) {
+#pragma unroll
+                for (int q = 0; q < UCH; q++) row0[c + q] = v0[q] - l0 * pv[q];
+            }
+            if (a1) {
+#pragma unroll
+                for (int q = 0; q < UCH; q++) row1[c + q] = v1[q] - l1 * pv[q];
             }
         }
-        block_argmax(v, r0, S.red_v, S.red_i, s_p);
-        __syncthreads();
-        const int p = *s_p;
-        TA piv = Aug[(size_t)p * ld + k];
-        if (piv == (TA)0) return false;  // same LDS value for all threads: uniform exit
-        if (tid == 0) {
-            S.perm[k] = p;
-            S.pos[p] = k;
+        for (; c < wcols; c++) {
+            TA pv = prow[c];
+            if (a0) row0[c] = row0[c] - l0 * pv;
+            if (a1) row1[c] = row1[c] - l1 * pv;
         }
-        TA inv = (TA)1 / piv;
-        // wave wv eliminates the unpivoted rows wv, wv+nw, ...; row p is excluded by index,
-        // so the concurrent write of pos[p] above is never read in this step
-        for (int r = wv; r < m; r += nw) {
-            if (r == p || S.pos[r] != m) continue;
-            TA l = Aug[(size_t)r * ld + k] * inv;
-            for (int c = k + 1 + lane; c < wcols; c += 64) Aug[(size_t)r * ld + c] -= l * Aug[(size_t)p * ld + c];
-            if (lane == 0) Aug[(size_t)r * ld + k] = l;
-        }
-        __syncthreads();
     }
     return true;
 }
 
-// Back substitution on the eliminated RHS columns; x_k overwrites Aug[perm[k]][m + t].
+// Back substitution on the eliminated RHS columns: the solution of unknown k overwrites
+// Aug[perm[k]][m + t]. Rows pivoted before step k subtract their U(., k) x_k.
 template <typename TA>
-__device__ void back_solve(Sys& S, TA* Aug) {
-    const int tid = threadIdx.x, nthr = blockDim.x;
-    const int m = S.m, ld = S.ld, T = S.T;
+__device__ void back_solve(const Sys& S, TA* Aug, const int (&posr)[2]) {
+    const int lane = threadIdx.x, m = S.m, ld = S.ld, T = S.T;
+    const int r0 = lane, r1 = lane + WAVE;
     for (int k = m - 1; k >= 0; k--) {
-        int p = S.perm[k];
-        if (tid < T) {
-            size_t e = (size_t)p * ld + m + tid;
-            Aug[e] = Aug[e] / Aug[(size_t)p * ld + k];
+        const int p = S.perm[k];
+        const TA ukk = Aug[p * ld + k];
+        for (int t = 0; t < T; t++) {
+            TA x = Aug[p * ld + m + t] / ukk;
+            if (r0 < m && posr[0] < k) Aug[r0 * ld + m + t] -= Aug[r0 * ld + k] * x;
+            if (r1 < m && posr[1] < k) Aug[r1 * ld + m + t] -= Aug[r1 * ld + k] * x;
+            if ((r0 == p) | (r1 == p)) Aug[p * ld + m + t] = x;
         }
-        __syncthreads();
-        for (int q = tid; q < k * T; q += nthr) {
-            int kk = q / T, t = q - kk * T;
-            int r = S.perm[kk];
-            Aug[(size_t)r * ld + m + t] -= Aug[(size_t)r * ld + k] * Aug[(size_t)p * ld + m + t];
-        }
-        __syncthreads();
     }
 }
 
-// Forward elimination of new RHS columns with the stored multipliers.
+// Forward elimination of new RHS columns (rows in original order) with the stored multipliers.
 template <typename TA>
-__device__ void forward_solve(Sys& S, TA* Aug) {
-    const int tid = threadIdx.x, nthr = blockDim.x;
-    const int m = S.m, ld = S.ld, T = S.T;
+__device__ void forward_solve(const Sys& S, TA* Aug, const int (&posr)[2]) {
+    const int lane = threadIdx.x, m = S.m, ld = S.ld, T = S.T;
+    const int r0 = lane, r1 = lane + WAVE;
     for (int k = 0; k < m - 1; k++) {
-        int p = S.perm[k];
-        for (int q = tid; q < m * T; q += nthr) {
-            int r = q / T, t = q - r * T;
-            if (S.pos[r] > k) Aug[(size_t)r * ld + m + t] -= Aug[(size_t)r * ld + k] * Aug[(size_t)p * ld + m + t];
+        const int p = S.perm[k];
+        for (int t = 0; t < T; t++) {
+            TA y = Aug[p * ld + m + t];
+            if (r0 < m && posr[0] > k) Aug[r0 * ld + m + t] -= Aug[r0 * ld + k] * y;
+            if (r1 < m && posr[1] > k) Aug[r1 * ld + m + t] -= Aug[r1 * ld + k] * y;
         }
-        __syncthreads();
     }
 }
 
 // r = b - M x in FP64 with M rebuilt from the coordinates; x(k,t) = XS[k*T+t] (by unknown).
-// Writes r into Aug's RHS columns (as TA) and returns, through red, max|r|, max|x|, max|b|
-// and ||M||_inf for the backward error. One thread per row.
-template <typename TA>
-__device__ void residual(Sys& S, TA* Aug, const double* XS, double* out_be) {
-    const int tid = threadIdx.x;
-    const int m = S.m, ld = S.ld, T = S.T;
-    double rmax[MAX_T], xmax[MAX_T], bmax[MAX_T];
+// With Aug set, r is written into its RHS columns (as TA). Returns the normwise backward
+// error max_t ||r_t|| / (||M|| ||x_t|| + ||b_t||) (max norms, ||M|| = max row sum).
+template <typename TA, int MT>
+__device__ double residual(const Sys& S, TA* Aug, const double* XS) {
+    const int lane = threadIdx.x, m = S.m, ld = S.ld, T = S.T;
+    double rmax[MT], xmax[MT], bmax[MT];
     for (int t = 0; t < T; t++) rmax[t] = xmax[t] = bmax[t] = 0.0;
     double rowsum = 0.0;
-    if (tid < m) {
-        double acc[MAX_T];
+    for (int h = 0; h < 2; h++) {
+        const int r = lane + h * WAVE;
+        if (r >= m) continue;
+        double acc[MT];
         for (int t = 0; t < T; t++) {
-            double b = S.BS ? S.BS[tid * T + t] : rhs_entry(S, tid, t);
+            double b = S.BS ? S.BS[r * T + t] : rhs_entry(S, r, t);
             acc[t] = b;
-            bmax[t] = fabs(b);
+            bmax[t] = fmax(bmax[t], fabs(b));
         }
+        double rs = 0.0;
         for (int c = 0; c < m; c++) {
-            double a = m_entry(tid, c, S.n, S.e2, S.X, S.P->sph);
-            rowsum += fabs(a);
+            double a = m_entry(r, c, S.n, S.e2, S.X, S.P->sph);
+            rs += fabs(a);
             for (int t = 0; t < T; t++) acc[t] -= a * XS[c * T + t];
         }
+        rowsum = fmax(rowsum, rs);
         for (int t = 0; t < T; t++) {
-            rmax[t] = fabs(acc[t]);
-            xmax[t] = fabs(XS[tid * T + t]);
-            if (Aug) Aug[(size_t)tid * ld + m + t] = (TA)acc[t];
+            rmax[t] = fmax(rmax[t], fabs(acc[t]));
+            xmax[t] = fmax(xmax[t], fabs(XS[r * T + t]));
+            if (Aug) Aug[r * ld + m + t] = (TA)acc[t];
         }
     }
-    // block max-reductions of rowsum and the per-column maxima
-    __shared__ double s_red[4 * MAX_T + 1];
-    if (tid < 3 * T + 1) s_red[tid] = 0.0;
-    __syncthreads();
-    for (int off = 32; off > 0; off >>= 1) {
-        rowsum = fmax(rowsum, __shfl_xor(rowsum, off, 64));
-        for (int t = 0; t < T; t++) {
-            rmax[t] = fmax(rmax[t], __shfl_xor(rmax[t], off, 64));
-            xmax[t] = fmax(xmax[t], __shfl_xor(xmax[t], off, 64));
-            bmax[t] = fmax(bmax[t], __shfl_xor(bmax[t], off, 64));
-        }
-    }
-    if ((tid & 63) == 0) {
-        // values are non-negative: integer max on the bit pattern is the float max
-        atomicMax((unsigned long long*)&s_red[0], (unsigned long long)__double_as_longlong(rowsum));
-        for (int t = 0; t < T; t++) {
-            atomicMax((unsigned long long*)&s_red[1 + t], (unsigned long long)__double_as_longlong(rmax[t]));
-            atomicMax((unsigned long long*)&s_red[1 + T + t], (unsigned long long)__double_as_longlong(xmax[t]));
-            atomicMax((unsigned long long*)&s_red[1 + 2 * T + t], (unsigned long long)__double_as_longlong(bmax[t]));
-        }
-    }
-    __syncthreads();
+    rowsum = wave_max(rowsum);
     double be = 0.0;
-    for (int t = 0; t < T; t++) be = fmax(be, s_red[1 + t] / (s_red[0] * s_red[1 + T + t] + s_red[1 + 2 * T + t]));
-    *out_be = be;
-    __syncthreads();
+    for (int t = 0; t < T; t++) be = fmax(be, wave_max(rmax[t]) / (rowsum * wave_max(xmax[t]) + wave_max(bmax[t])));
+    return be;
+}
+
+// max|U| over the factored rows, for the growth factor max|U| / max|M|.
+template <typename TA>
+__device__ double umax_of(const Sys& S, const TA* Aug, const int (&posr)[2]) {
+    const int lane = threadIdx.x, m = S.m, ld = S.ld;
+    double u = 0.0;
+    for (int h = 0; h < 2; h++) {
+        const int r = lane + h * WAVE;
+        if (r >= m) continue;
+        for (int c = posr[h]; c < m; c++) u = fmax(u, fabs((double)Aug[r * ld + c]));
+    }
+    return wave_max(u);
 }
 
 __device__ void stage_nodes(Sys& S) {
     const WParams& P = *S.P;
-    for (int j = threadIdx.x; j < S.n; j += blockDim.x) {
+    for (int j = threadIdx.x; j < S.n; j += WAVE) {
         const double* p = P.xyz + 3 * (size_t)P.st_idx[S.off + j];
         S.X[3 * j] = p[0];
         S.X[3 * j + 1] = p[1];
@@ -446,22 +471,37 @@ __device__ TA* setup(Sys& S, const WParams& P, unsigned char* smem, int s, bool 
         S.XS = S.BS = nullptr;
     }
     S.perm = (int*)q;
-    S.pos = S.perm + S.m;
-    S.red_v = (double*)(S.pos + S.m);
-    S.red_i = (int*)(S.red_v + 8);
     return (TA*)(smem + lds_head(S.m, S.n, S.T, ir));
 }
 
-__device__ void write_weights(const Sys& S, const WParams& P, const double* XS) {
-    for (int q = threadIdx.x; q < S.n * S.T; q += blockDim.x) {
-        int j = q / S.T, t = q - j * S.T;
-        double scale = (P.types[t] == RBFX_HV) ? P.hv_scale : 1.0;
-        P.w[(size_t)t * P.nnz + S.off + j] = XS[j * S.T + t] * scale;
+// Assembles [M | R] (M in TA, R in TA and, in FP32 mode, FP64 in BS) row by row, one row per
+// lane; returns max|M| over the wave.
+template <typename TA>
+__device__ double assemble(const Sys& S, TA* Aug) {
+    const int lane = threadIdx.x, m = S.m, ld = S.ld, T = S.T;
+    double amax = 0.0;
+    for (int h = 0; h < 2; h++) {
+        const int r = lane + h * WAVE;
+        if (r >= m) continue;
+        for (int c = 0; c < m; c++) {
+            double a = m_entry(r, c, S.n, S.e2, S.X, S.P->sph);
+            amax = fmax(amax, fabs(a));
+            Aug[r * ld + c] = (TA)a;
+        }
+        for (int t = 0; t < T; t++) {
+            double b = rhs_entry(S, r, t);
+            Aug[r * ld + m + t] = (TA)b;
+            if (S.BS) {
+                S.BS[r * T + t] = b;
+                S.XS[r * T + t] = 0.0;
+            }
+        }
     }
+    return wave_max(amax);
 }
 
 __device__ void write_failed(const Sys& S, const WParams& P) {
-    for (int q = threadIdx.x; q < S.n * S.T; q += blockDim.x) {
+    for (int q = threadIdx.x; q < S.n * S.T; q += WAVE) {
         int j = q / S.T, t = q - j * S.T;
         P.w[(size_t)t * P.nnz + S.off + j] = NAN;
     }
@@ -476,78 +516,52 @@ __device__ void write_failed(const Sys& S, const WParams& P) {
     }
 }
 
-// max|U| / max|M| over the factored rows (growth factor of the elimination).
-template <typename TA>
-__device__ double growth(const Sys& S, const TA* Aug) {
-    double umax = 0.0, amax = 0.0;
-    for (int q = threadIdx.x; q < S.m * S.m; q += blockDim.x) {
-        int r = q / S.m, c = q - r * S.m;
-        if (c >= S.pos[r]) umax = fmax(umax, fabs((double)Aug[(size_t)r * S.ld + c]));
-        amax = fmax(amax, fabs(m_entry(r, c, S.n, S.e2, S.X, S.P->sph)));
-    }
-    for (int off = 32; off > 0; off >>= 1) {
-        umax = fmax(umax, __shfl_xor(umax, off, 64));
-        amax = fmax(amax, __shfl_xor(amax, off, 64));
-    }
-    __shared__ double g_red[2];
-    if (threadIdx.x < 2) g_red[threadIdx.x] = 0.0;
-    __syncthreads();
-    if ((threadIdx.x & 63) == 0) {
-        atomicMax((unsigned long long*)&g_red[0], (unsigned long long)__double_as_longlong(umax));
-        atomicMax((unsigned long long*)&g_red[1], (unsigned long long)__double_as_longlong(amax));
-    }
-    __syncthreads();
-    double gr = g_red[0] / g_red[1];
-    __syncthreads();
-    return gr;
-}
-
 // FP64 path: assemble, factor, back-substitute (the arma::solve restatement on the GPU).
-__global__ void __launch_bounds__(WBLOCK) weights_f64_kernel(WParams P) {
+// MT bounds the number of right-hand sides (register arrays of the diagnostics).
+template <int MT>
+__global__ void __launch_bounds__(WAVE) weights_f64_kernel(WParams P) {
     extern __shared__ __align__(16) unsigned char smem[];
-    __shared__ int s_p;
     if ((int)blockIdx.x >= P.count) return;
     const int s = P.order[blockIdx.x];
     Sys S;
     double* Aug = setup<double>(S, P, smem, s, false);
-    const int tid = threadIdx.x, nthr = blockDim.x, m = S.m, T = S.T, ld = S.ld;
+    const int lane = threadIdx.x, m = S.m, T = S.T, ld = S.ld;
     stage_nodes(S);
-    __syncthreads();
-    for (int q = tid; q < m * m; q += nthr) {
-        int r = q / m, c = q - r * m;
-        Aug[(size_t)r * ld + c] = m_entry(r, c, S.n, S.e2, S.X, S.P->sph);
-    }
-    for (int q = tid; q < m * T; q += nthr) {
-        int r = q / T, t = q - r * T;
-        Aug[(size_t)r * ld + m + t] = rhs_entry(S, r, t);
-    }
-    __syncthreads();
-    if (!factor<double>(S, Aug, &s_p)) {
+    const double amax = assemble<double>(S, Aug);
+    int posr[2];
+    if (!factor<double>(S, Aug, posr)) {
         write_failed(S, P);
         return;
     }
-    back_solve<double>(S, Aug);
-    for (int q = tid; q < S.n * T; q += nthr) {
-        int j = q / T, t = q - j * T;
-        double scale = (P.types[t] == RBFX_HV) ? P.hv_scale : 1.0;
-        P.w[(size_t)t * P.nnz + S.off + j] = Aug[(size_t)S.perm[j] * ld + m + t] * scale;
+    back_solve<double>(S, Aug, posr);
+    for (int j = lane; j < S.n; j += WAVE) {
+        const int pj = S.perm[j];
+        for (int t = 0; t < T; t++) {
+            double scale = (P.types[t] == RBFX_HV) ? P.hv_scale : 1.0;
+            P.w[(size_t)t * P.nnz + S.off + j] = Aug[pj * ld + m + t] * scale;
+        }
     }
-    if (tid == 0) P.fail_flags[s] = 0;
+    if (lane == 0) P.fail_flags[s] = 0;
     if (P.diag) {
-        double gr = growth<double>(S, Aug);
-        // the factors are no longer needed: gather x by unknown into the head of Aug
-        // (through registers, so no thread overwrites a value another still has to read)
-        double* xs = Aug;
-        double tmp[MAX_T];
-        if (tid < m)
-            for (int t = 0; t < T; t++) tmp[t] = Aug[(size_t)S.perm[tid] * ld + m + t];
-        __syncthreads();
-        if (tid < m)
-            for (int t = 0; t < T; t++) xs[tid * T + t] = tmp[t];
-        __syncthreads();
-        double be;
-        residual<double>(S, (double*)nullptr, xs, &be);
-        if (tid == 0) {
+        const double gr = umax_of<double>(S, Aug, posr) / amax;
+        // x by unknown into the (no longer needed) coordinate-free head of Aug's row space:
+        // read all values first, then write, so no lane overwrites a value another still reads
+        double* xs = (double*)(smem + lds_head(m, S.n, T, false)) + (size_t)m * ld - (size_t)m * T;
+        double tmp[2][MT];
+        for (int h = 0; h < 2; h++) {
+            const int k = lane + h * WAVE;
+            if (k < m)
+                for (int t = 0; t < T; t++) tmp[h][t] = Aug[S.perm[k] * ld + m + t];
+        }
+        __builtin_amdgcn_wave_barrier();
+        for (int h = 0; h < 2; h++) {
+            const int k = lane + h * WAVE;
+            if (k < m)
+                for (int t = 0; t < T; t++) xs[k * T + t] = tmp[h][t];
+        }
+        __builtin_amdgcn_wave_barrier();
+        const double be = residual<double, MT>(S, (double*)nullptr, xs);
+        if (lane == 0) {
             P.diag[4 * (size_t)s] = be;
             P.diag[4 * (size_t)s + 1] = gr;
             P.diag[4 * (size_t)s + 2] = 0;
@@ -557,84 +571,67 @@ __global__ void __launch_bounds__(WBLOCK) weights_f64_kernel(WParams P) {
 }
 
 // FP32 path: FP32 factorisation of [M | R], then x += LU^-1 (b - M x) with the residual in
-// FP64 (M and b rebuilt in FP64) until the correction stops shrinking or is below FP64
-// resolution. A stencil that does not converge is flagged (fail_flags = 2) for an FP64 re-solve.
-__global__ void __launch_bounds__(WBLOCK) weights_f32ir_kernel(WParams P) {
+// FP64 (M rebuilt in FP64, b kept in FP64) until the correction stops shrinking or is below
+// FP64 resolution. A stencil that does not converge is flagged (fail_flags = 2) for an FP64
+// re-solve.
+template <int MT>
+__global__ void __launch_bounds__(WAVE) weights_f32ir_kernel(WParams P) {
     extern __shared__ __align__(16) unsigned char smem[];
-    __shared__ int s_p;
-    __shared__ double s_dn[2];
     if ((int)blockIdx.x >= P.count) return;
     const int s = P.order[blockIdx.x];
     Sys S;
     float* Aug = setup<float>(S, P, smem, s, true);
-    const int tid = threadIdx.x, nthr = blockDim.x, m = S.m, T = S.T, ld = S.ld;
+    const int lane = threadIdx.x, m = S.m, T = S.T, ld = S.ld;
     stage_nodes(S);
-    __syncthreads();
-    for (int q = tid; q < m * m; q += nthr) {
-        int r = q / m, c = q - r * m;
-        Aug[(size_t)r * ld + c] = (float)m_entry(r, c, S.n, S.e2, S.X, S.P->sph);
-    }
-    for (int q = tid; q < m * T; q += nthr) {
-        int r = q / T, t = q - r * T;
-        double b = rhs_entry(S, r, t);
-        S.BS[r * T + t] = b;
-        Aug[(size_t)r * ld + m + t] = (float)b;
-        S.XS[r * T + t] = 0.0;
-    }
-    __syncthreads();
-    if (!factor<float>(S, Aug, &s_p)) {
-        if (tid == 0) P.fail_flags[s] = 2;  // FP32 pivot underflow: let FP64 decide
+    const double amax = assemble<float>(S, Aug);
+    int posr[2];
+    if (!factor<float>(S, Aug, posr)) {
+        if (lane == 0) P.fail_flags[s] = 2;  // FP32 pivot underflow: let FP64 decide
         return;
     }
-    back_solve<float>(S, Aug);
+    back_solve<float>(S, Aug, posr);
     double prev = INFINITY, be = INFINITY;
     int it = 0;
     bool ok = false;
     for (;;) {
         // x += d, where d(k,t) = Aug[perm[k]][m+t]
         double dmax = 0.0, xmax = 0.0;
-        if (tid < m) {
-            int p = S.perm[tid];
+        for (int h = 0; h < 2; h++) {
+            const int k = lane + h * WAVE;
+            if (k >= m) continue;
+            const int p = S.perm[k];
             for (int t = 0; t < T; t++) {
-                double d = (double)Aug[(size_t)p * ld + m + t];
-                double x = S.XS[tid * T + t] + d;
-                S.XS[tid * T + t] = x;
+                double d = (double)Aug[p * ld + m + t];
+                double x = S.XS[k * T + t] + d;
+                S.XS[k * T + t] = x;
                 dmax = fmax(dmax, fabs(d));
                 xmax = fmax(xmax, fabs(x));
             }
         }
-        for (int off = 32; off > 0; off >>= 1) {
-            dmax = fmax(dmax, __shfl_xor(dmax, off, 64));
-            xmax = fmax(xmax, __shfl_xor(xmax, off, 64));
-        }
-        if (tid < 2) s_dn[tid] = 0.0;
-        __syncthreads();
-        if ((tid & 63) == 0) {
-            atomicMax((unsigned long long*)&s_dn[0], (unsigned long long)__double_as_longlong(dmax));
-            atomicMax((unsigned long long*)&s_dn[1], (unsigned long long)__double_as_longlong(xmax));
-        }
-        __syncthreads();
-        double rel = s_dn[0] / s_dn[1];
-        __syncthreads();
-        residual<float>(S, Aug, S.XS, &be);  // also loads r into the RHS columns
+        const double rel = wave_max(dmax) / wave_max(xmax);
+        be = residual<float, MT>(S, Aug, S.XS);  // also loads r into the RHS columns
         it++;
-        if (!(rel < 0.5 * prev) && it > 1) break;      // stagnation or divergence
+        if (!(rel < 0.5 * prev) && it > 1) break;  // stagnation or divergence
         if (rel <= 2.2e-16 || be <= 4.0 * 1.1e-16) { ok = true; break; }
         if (it >= P.max_iter) break;
         prev = rel;
-        forward_solve<float>(S, Aug);
-        back_solve<float>(S, Aug);
+        forward_solve<float>(S, Aug, posr);
+        back_solve<float>(S, Aug, posr);
     }
     ok = ok || be <= 4.0 * 1.1e-16 * m;
     if (!ok) {
-        if (tid == 0) P.fail_flags[s] = 2;
+        if (lane == 0) P.fail_flags[s] = 2;
         return;
     }
-    write_weights(S, P, S.XS);
-    if (tid == 0) P.fail_flags[s] = 0;
+    for (int j = lane; j < S.n; j += WAVE)
+        for (int t = 0; t < T; t++) {
+            double scale = (P.types[t] == RBFX_HV) ? P.hv_scale : 1.0;
+            P.w[(size_t)t * P.nnz + S.off + j] = S.XS[j * T + t] * scale;
+        }
+    if (lane == 0) P.fail_flags[s] = 0;
     if (P.diag) {
-        double gr = growth<float>(S, Aug);
-        if (tid == 0) {
+        const double gr = umax_of<float>(S, Aug, posr) / amax;
+        if (lane == 0) {
             P.diag[4 * (size_t)s] = be;
             P.diag[4 * (size_t)s + 1] = gr;
             P.diag[4 * (size_t)s + 2] = it;
@@ -760,10 +757,15 @@ extern "C" int rbfx_weights(rbfx_ctx* ctx, int32_t n_stencils, const int32_t* d_
             Q.fp32 = ir ? 1 : 0;
             size_t lds = ir ? lds_bytes<float>(Q.m, n, T, true) : lds_bytes<double>(Q.m, n, T, false);
             if (lds + STATIC_LDS_RESERVE > lds_limit) return rbfx_fail(ctx, RBFX_E_TOO_LARGE, "weights: system %d x %d with %d RHS needs %zu B of LDS", Q.m, Q.m, T, lds);
-            if (ir)
-                hipLaunchKernelGGL(weights_f32ir_kernel, dim3(c), dim3(WBLOCK), lds, st, Q);
+            const bool small_t = T <= 4;
+            if (ir && small_t)
+                hipLaunchKernelGGL(weights_f32ir_kernel<4>, dim3(c), dim3(WAVE), lds, st, Q);
+            else if (ir)
+                hipLaunchKernelGGL(weights_f32ir_kernel<MAX_T>, dim3(c), dim3(WAVE), lds, st, Q);
+            else if (small_t)
+                hipLaunchKernelGGL(weights_f64_kernel<4>, dim3(c), dim3(WAVE), lds, st, Q);
             else
-                hipLaunchKernelGGL(weights_f64_kernel, dim3(c), dim3(WBLOCK), lds, st, Q);
+                hipLaunchKernelGGL(weights_f64_kernel<MAX_T>, dim3(c), dim3(WAVE), lds, st, Q);
             hipError_t le = hipGetLastError();
             if (le != hipSuccess) return rbfx_fail(ctx, RBFX_E_HIP, "weights: launch (m=%d) failed: %s", Q.m, hipGetErrorString(le));
         }

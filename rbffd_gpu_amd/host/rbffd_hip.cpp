@@ -317,6 +317,23 @@ rbfx_ell* RBFFD_HIP::buildEll(DerTypes types) {
                              bits.data(), ptrs.data(), storage, &e);
     for (double* d : owned) rbfx_dev_free(ctx, d);
     if (rc != RBFX_OK) die(ctx, "rbfx_ell_create", rc);
+    // rows (stencil centres) processed along a Morton curve of their nodes: neighbouring
+    // stencils run side by side and share their gathers in cache; results are unchanged
+    std::vector<double> cxyz(3 * S);
+    for (size_t s = 0; s < S; s++) {
+        const NodeType& p = grid_ref.getNode((int)s);
+        cxyz[3 * s] = p.x();
+        cxyz[3 * s + 1] = p.y();
+        cxyz[3 * s + 2] = p.z();
+    }
+    std::vector<int32_t> order(S);
+    RBFX_CALL(ctx, rbfx_order_morton((int32_t)S, cxyz.data(), order.data()));
+    int32_t* d_order = nullptr;
+    RBFX_CALL(ctx, rbfx_dev_alloc(ctx, S * sizeof(int32_t), (void**)&d_order));
+    RBFX_CALL(ctx, rbfx_h2d(ctx, d_order, order.data(), S * sizeof(int32_t)));
+    rc = rbfx_ell_set_row_order(e, d_order);
+    rbfx_dev_free(ctx, d_order);
+    if (rc != RBFX_OK) die(ctx, "rbfx_ell_set_row_order", rc);
     return e;
 }
 

@@ -15,6 +15,7 @@ RBFX_OK = 0
 X, Y, Z, LAPL, R, HV, LAMBDA, THETA = 0x1, 0x2, 0x4, 0x8, 0x10, 0x20, 0x40, 0x80
 LSFC, XSFC, YSFC, ZSFC, INTERP = 0x100, 0x200, 0x400, 0x800, 0x8000
 F64, F32 = 0, 1
+F32_RN = 2  # weight storage: FP32 with plain nearest rounding (F32 keeps row sums)
 SOLVE_F64_LU, SOLVE_F32_LU_F64_IR = 0, 1
 POLY_SPHERE = 16  # poly_deg flag: sphere-reduced monomial basis
 PATH_F64, PATH_F32_IR, PATH_F32_FALLBACK_F64, PATH_SINGULAR = 0, 1, 2, 3
@@ -68,6 +69,10 @@ def load_library(path=None):
         "rbfx_knn_kdtree": (C.c_int, [i32, vp, i32, vp, i32]),
         "rbfx_nodes_fibonacci": (C.c_int, [i32, f64, u64, vp]),
         "rbfx_nodes_icosahedral": (C.c_int, [i32, vp, C.POINTER(i32)]),
+        "rbfx_order_morton": (C.c_int, [i32, vp, vp]),
+        "rbfx_ell_set_row_order": (C.c_int, [vp, vp]),
+        "rbfx_ell_build_local": (C.c_int, [vp, i32, i32]),
+        "rbfx_ell_local_info": (C.c_int, [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)]),
         "rbfx_weights": (C.c_int, [vp, i32, vp, vp, i32, vp, vp, u32, i32, i32, i32, f64, i32, vp, vp, C.POINTER(i32)]),
         "rbfx_ell_create": (C.c_int, [vp, i32, i32, i32, vp, i32, vp, vp, i32, C.POINTER(vp)]),
         "rbfx_ell_destroy": (None, [vp]),
@@ -80,8 +85,9 @@ def load_library(path=None):
         "rbfx_rk4_get_state": (C.c_int, [vp, vp]),
         "rbfx_rk4_step": (C.c_int, [vp, f64, i32]),
         "rbfx_rk4_stage": (C.c_int, [vp, i32, f64]),
-        "rbfx_tune_set": (C.c_int, [vp, i32, i32, i32, i32, i32, i32, i32]),
-        "rbfx_tune_get": (C.c_int, [vp, i32, i32, i32, i32, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)]),
+        "rbfx_tune_set": (C.c_int, [vp, i32, i32, i32, i32, i32, i32, i32, i32]),
+        "rbfx_tune_get": (C.c_int, [vp, i32, i32, i32, i32, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32),
+                                    C.POINTER(i32)]),
         "rbfx_tune_clear": (C.c_int, []),
         "rbfx_cosine_bell_coefs": (C.c_int, [vp, i32, vp, f64, f64, f64, vp, vp]),
         "rbfx_cosine_bell_init": (C.c_int, [vp, i32, vp, f64, vp, i32]),
@@ -139,6 +145,16 @@ def nodes_icosahedral(level):
     rc = RBFX.rbfx_nodes_icosahedral(level, _ptr(out), C.byref(n))
     if rc:
         raise RbfxError(rc, "nodes_icosahedral failed")
+    return out
+
+
+def order_morton(xyz):
+    """Node order along a 3-D Morton curve (int32 permutation), for EllOperators.set_row_order."""
+    xyz = np.ascontiguousarray(xyz, np.float64)
+    out = np.empty(xyz.shape[0], np.int32)
+    rc = RBFX.rbfx_order_morton(xyz.shape[0], _ptr(xyz), _ptr(out))
+    if rc:
+        raise RbfxError(rc, "order_morton failed")
     return out
 
 
@@ -314,7 +330,8 @@ class EllOperators:
                                   C.byref(h))
         ctx.check(rc, "ell_create")
         self.h = h
-        self.n_rows, self.width, self.n_ops, self.storage = n_rows, width, n_ops, storage
+        self.n_rows, self.width, self.n_ops = n_rows, width, n_ops
+        self.storage = F32 if storage == F32_RN else storage
 
     def spmv(self, op, d_u, d_out, field_type=F64, accum=F64):
         self.ctx.check(RBFX.rbfx_spmv(self.h, op, d_u.ptr, d_out.ptr, field_type, accum), "spmv")
@@ -322,14 +339,32 @@ class EllOperators:
     def spmm(self, n_fields, d_u, d_out, field_type=F64, accum=F64):
         self.ctx.check(RBFX.rbfx_spmm(self.h, n_fields, d_u.ptr, d_out.ptr, field_type, accum), "spmm")
 
-    def tune_set(self, kind, n_fields, field_type, accum, fpt, block, batch=4):
-        self.ctx.check(RBFX.rbfx_tune_set(self.h, kind, n_fields, field_type, accum, fpt, block, batch), "tune_set")
+    def tune_set(self, kind, n_fields, field_type, accum, fpt, block, batch=4, share=0):
+        self.ctx.check(RBFX.rbfx_tune_set(self.h, kind, n_fields, field_type, accum, fpt, block, batch, share), "tune_set")
 
     def tune_get(self, kind, n_fields, field_type, accum):
-        f, b, u = C.c_int32(), C.c_int32(), C.c_int32()
-        self.ctx.check(RBFX.rbfx_tune_get(self.h, kind, n_fields, field_type, accum, C.byref(f), C.byref(b), C.byref(u)),
-                       "tune_get")
-        return f.value, b.value, u.value
+        """(fields per thread, block, batch, share) the library will launch with."""
+        f, b, u, sh = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
+        self.ctx.check(RBFX.rbfx_tune_get(self.h, kind, n_fields, field_type, accum, C.byref(f), C.byref(b), C.byref(u),
+                                          C.byref(sh)), "tune_get")
+        return f.value, b.value, u.value, sh.value
+
+    def set_row_order(self, order):
+        """Process rows in `order` (a permutation; host array or device array); results unchanged."""
+        d = order if isinstance(order, (DeviceArray, DeviceView)) else self.ctx.to_device(np.asarray(order, np.int32))
+        self.ctx.check(RBFX.rbfx_ell_set_row_order(self.h, d.ptr), "ell_set_row_order")
+        self.ctx.sync()
+
+    def build_local(self, tile_rows=64, max_nodes=0):
+        """Tables for LDS-staged gathers (tiles of at most tile_rows positions and max_nodes
+        distinct nodes); returns (tile_rows, k_max, n_tiles)."""
+        self.ctx.check(RBFX.rbfx_ell_build_local(self.h, tile_rows, max_nodes), "ell_build_local")
+        return self.local_info()
+
+    def local_info(self):
+        t, k, n = C.c_int32(), C.c_int32(), C.c_int32()
+        self.ctx.check(RBFX.rbfx_ell_local_info(self.h, C.byref(t), C.byref(k), C.byref(n)), "ell_local_info")
+        return t.value, k.value, n.value
 
     def close(self):
         if self.h:

@@ -1,6 +1,7 @@
 """Autotuner for the fused ELL kernels (SpMM and RK4 stage).
 
-Sweeps the launch shapes the library offers (fields per thread, block size, gather batch),
+Sweeps the launch shapes the library offers (fields per thread, block size, gather batch
+and, for the RK4 stage kernel, whether the lanes of a row share the index/weight loads),
 times each with HIP events on the context stream, installs the fastest with rbfx_tune_set
 and keeps a small JSON table (keyed by rows, width, operators, fields, precisions) so a later
 run can reuse the choice without sweeping.
@@ -11,18 +12,40 @@ import os
 from .lib import RK4, EllOperators
 
 KIND_SPMM, KIND_RK4 = 0, 1
+LDS_BYTES = 160 * 1024  # LDS per workgroup on gfx950
 DEFAULT_TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning_table.json")
 
 
-def candidates(n_fields):
+def _pipe_ok(n_ops, fpt, G, share):
+    """Pipelined shared-load cores the library instantiates (share 2/3 = prefetch depth)."""
+    return n_ops >= 3 and ((fpt == 4 and G == 4) or (fpt == 8 and G == 2) or
+                           (share == 2 and ((fpt == 4 and G == 2) or (fpt == 2 and G == 4))))
+
+
+def candidates(n_fields, kind=KIND_SPMM, n_ops=1, local=None, field_bytes=8):
+    """(fpt, block, batch, share) shapes the library accepts for n_fields fields.
+    local = (tile_rows, k_max) of an operator set with LDS-staging tables: adds the staged
+    shapes (share 4; the block is the tile) when a tile's records fit in LDS_BYTES."""
+    if kind == KIND_RK4 and local and local[0] > 0 and n_ops >= 3 and (n_fields * field_bytes) % 16 == 0 \
+            and local[1] * n_fields * field_bytes <= LDS_BYTES:
+        for fpt, G in ((4, 4), (8, 2)):
+            if n_fields == fpt * G and local[0] * G <= 256:
+                for batch in (4, 8):
+                    yield fpt, local[0] * G, batch, 4
     for fpt in (1, 2, 4, 8):
         if n_fields % fpt:
             continue
+        G = n_fields // fpt
         for block in (64, 128, 256):
-            if n_fields // fpt > block:
+            if G > block:
                 continue
             for batch in (4, 8):
-                yield fpt, block, batch
+                yield fpt, block, batch, 0
+                if kind == KIND_RK4 and fpt > 1 and G in (2, 4) and batch % G == 0:
+                    yield fpt, block, batch, 1
+                    for share in (2, 3):
+                        if _pipe_ok(n_ops, fpt, G, share):
+                            yield fpt, block, batch, share
 
 
 def _key(ell, kind, n_fields, field_type, accum):
@@ -44,7 +67,7 @@ def save_table(table, path=DEFAULT_TABLE):
 def apply_table(ell: EllOperators, kind, n_fields, field_type, accum, table):
     e = table.get(_key(ell, kind, n_fields, field_type, accum))
     if e:
-        ell.tune_set(kind, n_fields, field_type, accum, e["fpt"], e["block"], e["batch"])
+        ell.tune_set(kind, n_fields, field_type, accum, e["fpt"], e["block"], e["batch"], e.get("share", 0))
         return e
     return None
 
@@ -53,8 +76,9 @@ def tune_rk4(ctx, rk: RK4, ell: EllOperators, field_type, accum, dt=1e-6, reps=3
     """Times one RK4 step (4 eager stage launches) per shape; installs and returns the best."""
     F = rk.n_fields
     results = []
-    for fpt, block, batch in candidates(F):
-        ell.tune_set(KIND_RK4, F, field_type, accum, fpt, block, batch)
+    local = ell.local_info()
+    for fpt, block, batch, share in candidates(F, KIND_RK4, ell.n_ops, local, 8 if field_type == 0 else 4):
+        ell.tune_set(KIND_RK4, F, field_type, accum, fpt, block, batch, share)
         for s in (1, 2, 3, 4):
             rk.stage(s, dt)  # warm
         ctx.timer_start()
@@ -62,12 +86,12 @@ def tune_rk4(ctx, rk: RK4, ell: EllOperators, field_type, accum, dt=1e-6, reps=3
             for s in (1, 2, 3, 4):
                 rk.stage(s, dt)
         ms = ctx.timer_stop() / reps
-        results.append((ms, fpt, block, batch))
+        results.append((ms, fpt, block, batch, share))
         if log:
-            log(f"  rk4 shape fpt={fpt} block={block} batch={batch}: {ms * 1e3:.1f} us/step")
-    ms, fpt, block, batch = min(results)
-    ell.tune_set(KIND_RK4, F, field_type, accum, fpt, block, batch)
-    entry = {"fpt": fpt, "block": block, "batch": batch, "us_per_step": round(ms * 1e3, 2)}
+            log(f"  rk4 shape fpt={fpt} block={block} batch={batch} share={share}: {ms * 1e3:.1f} us/step")
+    ms, fpt, block, batch, share = min(results)
+    ell.tune_set(KIND_RK4, F, field_type, accum, fpt, block, batch, share)
+    entry = {"fpt": fpt, "block": block, "batch": batch, "share": share, "us_per_step": round(ms * 1e3, 2)}
     if table is not None:
         table[_key(ell, KIND_RK4, F, field_type, accum)] = entry
     return entry, sorted(results)
@@ -75,8 +99,8 @@ def tune_rk4(ctx, rk: RK4, ell: EllOperators, field_type, accum, dt=1e-6, reps=3
 
 def tune_spmm(ctx, ell: EllOperators, n_fields, d_u, d_out, field_type, accum, reps=5, table=None, log=None):
     results = []
-    for fpt, block, batch in candidates(n_fields):
-        ell.tune_set(KIND_SPMM, n_fields, field_type, accum, fpt, block, batch)
+    for fpt, block, batch, share in candidates(n_fields, KIND_SPMM):
+        ell.tune_set(KIND_SPMM, n_fields, field_type, accum, fpt, block, batch, share)
         ell.spmm(n_fields, d_u, d_out, field_type, accum)
         ctx.timer_start()
         for _ in range(reps):
@@ -86,7 +110,7 @@ def tune_spmm(ctx, ell: EllOperators, n_fields, d_u, d_out, field_type, accum, r
         if log:
             log(f"  spmm shape fpt={fpt} block={block} batch={batch}: {ms * 1e3:.1f} us")
     ms, fpt, block, batch = min(results)
-    ell.tune_set(KIND_SPMM, n_fields, field_type, accum, fpt, block, batch)
+    ell.tune_set(KIND_SPMM, n_fields, field_type, accum, fpt, block, batch, 0)
     entry = {"fpt": fpt, "block": block, "batch": batch, "us": round(ms * 1e3, 2)}
     if table is not None:
         table[_key(ell, KIND_SPMM, n_fields, field_type, accum)] = entry

@@ -145,3 +145,80 @@ def test_ell_rejects_out_of_range_index(ctx):
     idx[7, 3] = 100
     with pytest.raises(R.RbfxError):
         R.EllOperators(ctx, idx, [np.ones((100, 5))])
+
+
+def test_f32_storage_rounding_matches_oracle(ctx, c1):
+    """FP32 weight storage: the row-sum-preserving rounding is bit-identical to the CPU
+    restatement (applied with FP64 fields and accumulation, any differing weight would show
+    at the 1e-8 level), RBFX_F32_RN is the plain cast, and the stored row sums of the
+    derivative operators stay near zero."""
+    print("case: fp32 weight rounding", flush=True)
+    xyz, st = c1["xyz"], c1["st"]
+    off, idx = R.csr_from_fixed(st)
+    w, _, _ = O.weights(off, idx, xyz, 3.5, R.X | R.Y | R.Z | R.LSFC, dim=3, poly_deg=2, diag=True)
+    ws = [w[t].reshape(st.shape) for t in range(4)]
+    rng = np.random.default_rng(3)
+    u = rng.standard_normal(st.shape[0])
+    d_u = ctx.to_device(u)
+    ones = ctx.to_device(np.ones(st.shape[0]))
+    d_o = ctx.alloc(st.shape[0], np.float64)
+    for storage, rounder in ((R.F32, O.round_f32_rowsum), (R.F32_RN, lambda a: a.astype(np.float32))):
+        ell = R.EllOperators(ctx, st, ws, storage=storage)
+        for o in range(4):
+            w32 = rounder(ws[o]).astype(np.float64)
+            ell.spmv(o, d_u, d_o, R.F64, R.F64)
+            ref = O.apply(off, idx, w32.reshape(-1), u)
+            assert rel_apply_err(d_o.download(), ref, _absum(st, w32, u)) <= TOL64, (storage, o)
+            ell.spmv(o, ones, d_o, R.F64, R.F64)
+            rs = np.abs(d_o.download())
+            if storage == R.F32:
+                near = np.abs(ws[o].astype(np.float32).astype(np.float64).sum(1))
+                print(f"  op {o}: max |row sum| stored {rs.max():.2e}, nearest rounding {near.max():.2e}")
+                assert rs.max() <= 0.1 * near.max()
+        ell.close()
+
+
+def test_row_order_bit_identical(ctx):
+    """A processing order (Morton locality order or a random permutation) changes which rows run
+    together, not the results: SpMM, single-operator SpMV and the strided SpMV are bit-identical
+    to the node-order layout, and a non-permutation is rejected."""
+    print("case: row order", flush=True)
+    rng = np.random.default_rng(21)
+    xyz = R.nodes_fibonacci(3000, 1e-3, 4)
+    st = R.knn_kdtree(xyz, 31)
+    n_rows, width = st.shape
+    ws = [rng.standard_normal((n_rows, width)) for _ in range(3)]
+    F = 8
+    u = rng.standard_normal((n_rows, F))
+    d_u = ctx.to_device(u)
+    d_u1 = ctx.to_device(u[:, 0].copy())
+    base = R.EllOperators(ctx, st, ws, storage=R.F64)
+    d_ref = ctx.alloc((3, n_rows, F), np.float64)
+    base.spmm(F, d_u, d_ref)
+    ref = d_ref.download()
+    d_ref1 = ctx.alloc(n_rows, np.float64)
+    base.spmv(1, d_u1, d_ref1)
+    ref1 = d_ref1.download()
+    for order in (R.order_morton(xyz), rng.permutation(n_rows).astype(np.int32)):
+        ell = R.EllOperators(ctx, st, ws, storage=R.F64)
+        ell.set_row_order(order)
+        ell.set_row_order(order[::-1].copy())  # a second re-layout starts from the first
+        d_o = ctx.alloc((3, n_rows, F), np.float64)
+        ell.spmm(F, d_u, d_o)
+        assert np.array_equal(d_o.download(), ref)
+        d_o1 = ctx.alloc(n_rows, np.float64)
+        ell.spmv(1, d_u1, d_o1)
+        assert np.array_equal(d_o1.download(), ref1)
+        ell.close()
+    single = R.EllOperators(ctx, st, [ws[2]], storage=R.F64)
+    single.set_row_order(R.order_morton(xyz))
+    d_o1 = ctx.alloc(n_rows, np.float64)
+    single.spmv(0, d_u1, d_o1)
+    base.spmv(2, d_u1, d_ref1)
+    assert np.array_equal(d_o1.download(), d_ref1.download())
+    bad = np.arange(n_rows, dtype=np.int32)
+    bad[5] = 4
+    with pytest.raises(R.RbfxError):
+        single.set_row_order(bad)
+    single.close()
+    base.close()

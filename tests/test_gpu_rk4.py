@@ -69,6 +69,45 @@ def test_rk4_graph_equals_eager(ctx, c1):
     assert np.array_equal(outs[0], outs[1])
 
 
+@pytest.mark.parametrize("storage", [R.F64, R.F32])
+def test_rk4_shared_loads_bit_identical(ctx, c1, storage):
+    """The shared index/weight loads (lanes of a row split the loads, DPP broadcast) keep the
+    summation order, so every launch shape gives bit-identical states; the FP64 path also
+    matches the CPU restatement."""
+    print(f"case: rk4 shared loads storage={storage}", flush=True)
+    xyz, st, ws = _c1_ops(c1)
+    F = 16
+    u0 = _fields(xyz, F, seed=5)
+    om = np.array([0.3, -0.2, 1.0])
+    coefs = [-(om[1] * xyz[:, 2] - om[2] * xyz[:, 1]), -(om[2] * xyz[:, 0] - om[0] * xyz[:, 2]),
+             -(om[0] * xyz[:, 1] - om[1] * xyz[:, 0]), None]
+    ell = R.EllOperators(ctx, st, ws, storage=storage)
+    ell.set_row_order(R.order_morton(xyz))
+    tl, k_max, n_tiles = ell.build_local(64, 128)  # the node cap shortens tiles: variable tile sizes
+    print(f"  LDS tiles of <= {tl} rows, at most {k_max} distinct nodes, {n_tiles} tiles")
+    assert k_max <= 128 and n_tiles > (st.shape[0] + 63) // 64
+    rk = R.RK4(ctx, ell, F, storage, R.F64, coefs=coefs)
+    outs = {}
+    from rbffd_gpu_amd import tune as TU
+    for fpt, block, batch, share in TU.candidates(F, TU.KIND_RK4, ell.n_ops, (tl, k_max, n_tiles),
+                                                  8 if storage == R.F64 else 4):
+        if block != 128 and share != 4:
+            continue
+        ell.tune_set(TU.KIND_RK4, F, storage, R.F64, fpt, block, batch, share)
+        rk.set_state(u0)
+        rk.step(1e-3, 3)
+        outs[(fpt, batch, share)] = rk.get_state()
+    assert any(k[2] for k in outs) and any(k[2] == 4 for k in outs)
+    base = outs[(4, 8, 0)]
+    for k, v in outs.items():
+        assert np.array_equal(v, base), k
+    if storage == R.F64:
+        ref = O.rk4_step(st, ws, coefs, u0, 1e-3, n_steps=3)
+        assert np.abs(base - ref).max() <= 1e-12 * np.abs(ref).max()
+    rk.close()
+    ell.close()
+
+
 def test_rk4_amplification_factor(ctx):
     """du/dt = lambda u through a diagonal operator: one step multiplies u by
     R(z) = 1 + z + z^2/2 + z^3/6 + z^4/24, z = lambda dt (closed form of classic RK4)."""
@@ -141,3 +180,27 @@ def test_cosine_bell_rhs_and_step(ctx, c1):
     assert np.abs(got - ref).max() / np.abs(ref).max() <= 1e-12
     rk.close()
     ell.close()
+
+
+def test_rk4_row_order_bit_identical(ctx, c1):
+    """RK4 on an operator set with a Morton processing order equals the node-order run bit for
+    bit, including a graph captured before the re-layout (it is recaptured)."""
+    xyz, st, ws = _c1_ops(c1)
+    F = 16
+    u0 = _fields(xyz, F, seed=8)
+    om = np.array([0.3, -0.2, 1.0])
+    coefs = [-(om[1] * xyz[:, 2] - om[2] * xyz[:, 1]), -(om[2] * xyz[:, 0] - om[0] * xyz[:, 2]),
+             -(om[0] * xyz[:, 1] - om[1] * xyz[:, 0]), None]
+    outs = []
+    for ordered in (False, True):
+        ell = R.EllOperators(ctx, st, ws, storage=R.F32)
+        rk = R.RK4(ctx, ell, F, R.F32, R.F64, coefs=coefs)
+        rk.set_state(u0)
+        rk.step(1e-3, 2)
+        if ordered:
+            ell.set_row_order(R.order_morton(xyz))
+        rk.step(1e-3, 2)
+        outs.append(rk.get_state())
+        rk.close()
+        ell.close()
+    assert np.array_equal(outs[0], outs[1])

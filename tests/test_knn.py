@@ -75,3 +75,15 @@ def test_icosahedral_nodes():
         assert x.shape[0] == 10 * 4 ** lev + 2
         assert np.allclose(np.linalg.norm(x, axis=1), 1.0, atol=1e-15)
         assert len(np.unique(np.round(x, 12), axis=0)) == x.shape[0]
+
+
+def test_order_morton_is_local_permutation():
+    """The Morton order is a permutation, deterministic, and puts spatial neighbours next to
+    each other: consecutive nodes are far closer than in the spiral's own order."""
+    xyz = R.nodes_fibonacci(20000, 1e-3, 0)
+    o = R.order_morton(xyz)
+    assert np.array_equal(np.sort(o), np.arange(20000))
+    assert np.array_equal(o, R.order_morton(xyz))
+    step_sorted = np.median(np.linalg.norm(np.diff(xyz[o], axis=0), axis=1))
+    step_spiral = np.median(np.linalg.norm(np.diff(xyz, axis=0), axis=1))
+    assert step_sorted < 0.1 * step_spiral

@@ -311,3 +311,33 @@ def test_sphere_reduced_basis_reproduces_degree2_harmonics():
         Y = real_sph_harm(l, m, xyz)
         d = O.apply(off, idx, w[0], Y)
         assert np.abs(d + l * (l + 1) * Y).max() < 1e-6 * l * (l + 1), (l, m)
+
+
+def test_round_f32_rowsum_properties():
+    """Pins the FP32 row-sum-preserving rounding by what it must satisfy: every value within
+    1.5 FP32 ulp of the FP64 weight, zero stays zero, a row whose nearest-rounded values
+    already sum exactly is left as the plain cast, and on RBF-FD derivative rows (largest
+    weight first) the stored row sum is orders of magnitude closer to the FP64 one."""
+    rng = np.random.default_rng(7)
+    W = rng.standard_normal((500, 40)) * np.exp(rng.uniform(-6, 6, (500, 40)))
+    W[:, -3:] = 0.0  # padding slots
+    w32 = O.round_f32_rowsum(W)
+    ulp = np.spacing(np.abs(W.astype(np.float32))).astype(np.float64)
+    assert np.all(np.abs(w32.astype(np.float64) - W) <= 1.5 * ulp)
+    assert np.all(w32[:, -3:] == 0.0)
+    Wi = rng.integers(-100, 100, (50, 20)).astype(np.float64) / 8.0  # exact in FP32: no moves
+    assert np.array_equal(O.round_f32_rowsum(Wi), Wi.astype(np.float32))
+    xyz = R.nodes_fibonacci(2048, 1e-3, 3)
+    st = R.knn_kdtree(xyz, 31)
+    off, idx = R.csr_from_fixed(st)
+    w, _, _ = O.weights(off, idx, xyz, 3.0, R.X | R.LSFC, dim=3, poly_deg=2)
+    for t in range(2):
+        Wt = w[t].reshape(st.shape)
+        S = Wt.sum(1)
+        near = np.abs(Wt.astype(np.float32).astype(np.float64).sum(1) - S)
+        kept = np.abs(O.round_f32_rowsum(Wt).astype(np.float64).sum(1) - S)
+        assert np.median(kept) < 1e-3 * np.median(near)
+        # a planted mistake (moving by two ulps instead of one) breaks the 1.5-ulp bound
+        bad = O.round_f32_rowsum(Wt).astype(np.float64)
+        bad[:, 1] += 2 * np.spacing(np.abs(Wt[:, 1]).astype(np.float32))
+        assert np.any(np.abs(bad - Wt) > 1.5 * np.spacing(np.abs(Wt.astype(np.float32))).astype(np.float64))
