@@ -108,7 +108,7 @@ __global__ void __launch_bounds__(256) rk4_stage_lds_kernel(StageArgs a) {
     const int row = ell_row_at(a.order, p);
     const int fofs = (t - rl * SG) * FPT;
     AT d[NOPS][FPT];
-    ell_gather_dot_lds<WT, UT, AT, NOPS, FPT, UB, SG, PD>(a.loc_idx, (const WT*)a.w_t, lds, p, a.width, a.F, fofs,
+    ell_gather_dot_lds<WT, UT, AT, NOPS, FPT, UB, SG, PD>(a.loc_idx, (const WT*)a.w_t, lds, p, a.width, fofs,
                                                           t - rl * SG, d);
     stage_update<UT, AT, NOPS, FPT>(a, row, fofs, d);
 }
