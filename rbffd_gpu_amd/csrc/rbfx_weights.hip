@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <cstdlib>
 #include <vector>
 
 #include "rbfx_internal.h"
@@ -570,6 +571,355 @@ __global__ void __launch_bounds__(WAVE) weights_f64_kernel(WParams P) {
     }
 }
 
+// ---- register-resident elimination (FP64) -------------------------------------------------
+// The augmented matrix [M | R] lives in registers: lane L of each of the W waves of a block
+// holds rows L and L + 64 (R = 2; R = 1 when m <= 64) of the columns c = W*j + w that wave w
+// owns (slot j = 0..NJ-1, column-cyclic; slots past m + T hold zeros), so a rank-1 update is
+// R FMAs per owned column and lane, with no LDS traffic for the matrix. Elimination step k:
+//   * column k, stored to LDS by its owner wave at the end of step k-1, is read by every
+//     wave, which finds the pivot itself (a DPP max over packed keys: same data, same result);
+//   * the pivot lane of each wave copies its row's slots to a per-wave LDS row buffer, which
+//     the wave reads back as 16-byte broadcasts;
+//   * the update runs over groups of 8 slots, skipping the groups that hold only columns
+//     <= k. Columns <= k of unpivoted rows are dead (the pivot search and the multipliers only
+//     read column k), so the slots of a live group are updated without a per-column test;
+//     pivoted rows have multiplier 0 and keep their values;
+//   * the owner wave of column k+1 stores the update group that holds it (slots with
+//     compile-time indices; the readers pick the column's slot at run time), double-buffered;
+//     one block barrier ends the step.
+// After the last step each lane stores its row's U part (columns > its pivot step) and its
+// eliminated right-hand sides to LDS; back substitution then runs in pivot order, each wave on
+// its own right-hand sides with lane i holding unknown i. Every live element sees the
+// operations of the one-wave LDS kernel in the same order, so the two give bit-identical
+// weights.
+constexpr int REG_G = 8;  // slots per update group
+
+// LDS: [pivot rows: W*NJ][column groups: 2 x rows x REG_G][X: 3n][SPH: 2n][pivots: m]
+// [U: m(m+1)/2][BS: m*T] doubles (rows = 64 R); column c of U (rows 0..c in pivot order)
+// starts at c(c+1)/2.
+__host__ __device__ inline size_t reg_lds_bytes(int m, int n, int T, int wnj, int rows) {
+    return ((size_t)wnj + 2 * (size_t)rows * REG_G + (size_t)n * 5 + m + (size_t)m * (m + 1) / 2 + (size_t)m * T) *
+           sizeof(double);
+}
+
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ unsigned long long umax_dpp_step(unsigned long long v) {
+    const int lo = (int)(unsigned)v, hi = (int)(unsigned)(v >> 32);
+    const unsigned olo = (unsigned)__builtin_amdgcn_update_dpp(lo, lo, CTRL, ROWMASK, 0xf, false);
+    const unsigned ohi = (unsigned)__builtin_amdgcn_update_dpp(hi, hi, CTRL, ROWMASK, 0xf, false);
+    const unsigned long long o = ((unsigned long long)ohi << 32) | olo;
+    return o > v ? o : v;
+}
+
+// Maximum over the 64 lanes as a wave-uniform value (all lanes must be active): two quad
+// swaps and two row rotations give every lane its 16-lane row's maximum, two row broadcasts
+// fold rows 0..2 into row 3, and lane 63 is read.
+__device__ __forceinline__ unsigned long long wave_umax_uniform(unsigned long long v) {
+    v = umax_dpp_step<0xB1, 0xf>(v);   // quad_perm [1,0,3,2]
+    v = umax_dpp_step<0x4E, 0xf>(v);   // quad_perm [2,3,0,1]
+    v = umax_dpp_step<0x124, 0xf>(v);  // row_ror:4
+    v = umax_dpp_step<0x128, 0xf>(v);  // row_ror:8
+    v = umax_dpp_step<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+    v = umax_dpp_step<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)v, 63);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(v >> 32), 63);
+    return ((unsigned long long)hi << 32) | lo;
+}
+
+__device__ __forceinline__ double readlane_f64(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)b, l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// Makes this wave's earlier LDS stores visible to its later LDS loads in other lanes: without
+// it the compiler may run the loads of lanes that did not store ahead of the stores of the one
+// that did (lanes of a wave do not synchronise in the memory model).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Elimination step with the pivot row in slot H of lane pl; the groups below G0 hold only
+// columns <= k (a compile-time bound, so every update is in place). The owner wave of column
+// k+1 (own) stores group G0, which holds that column, row by row into cg.
+template <int R, int NJ, int H, int G0>
+__device__ __forceinline__ void reg_step(double (&a)[R][NJ], const double (&l)[R], bool own, int lane, int pl,
+                                         double2* rb, double* cg) {
+    constexpr int NG = NJ / REG_G, GQ = REG_G / 2;
+    wave_lds_sync();  // the previous step's row reads are done before the row is replaced
+    if (lane == pl) {
+#pragma unroll
+        for (int q = G0 * GQ; q < NJ / 2; q++) rb[q] = make_double2(a[H][2 * q], a[H][2 * q + 1]);
+    }
+    wave_lds_sync();
+    // the row values of group g+1 are loaded while group g is updated; the scheduling
+    // barriers keep the loads of later groups from being hoisted (one group of lookahead)
+    double2 un[GQ];
+#pragma unroll
+    for (int q = 0; q < GQ; q++) un[q] = rb[G0 * GQ + q];
+#pragma unroll
+    for (int g = G0; g < NG; g++) {
+        double2 u[GQ];
+#pragma unroll
+        for (int q = 0; q < GQ; q++) u[q] = un[q];
+        if (g + 1 < NG) {
+#pragma unroll
+            for (int q = 0; q < GQ; q++) un[q] = rb[(g + 1) * GQ + q];
+        }
+#pragma unroll
+        for (int q = 0; q < GQ; q++) {
+            const int j = g * REG_G + 2 * q;
+#pragma unroll
+            for (int h = 0; h < R; h++) {
+                a[h][j] = a[h][j] - l[h] * u[q].x;
+                a[h][j + 1] = a[h][j + 1] - l[h] * u[q].y;
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    if (own) {
+#pragma unroll
+        for (int h = 0; h < R; h++) {
+            double2* dst = (double2*)(cg + (lane + WAVE * h) * REG_G);
+#pragma unroll
+            for (int q = 0; q < GQ; q++) dst[q] = make_double2(a[h][G0 * REG_G + 2 * q], a[h][G0 * REG_G + 2 * q + 1]);
+        }
+    }
+}
+
+// Elimination steps of phase G: the steps k with W*REG_G*G <= k + 1 - w < W*REG_G*(G+1), in
+// which the groups below G hold only dead columns (phases differ between waves; every step
+// ends with one block barrier in every wave). Returns false on an exact zero pivot.
+template <int W, int R, int NJ, int G>
+__device__ __forceinline__ bool reg_phase(double (&a)[R][NJ], int (&posr)[R], int& k, int m, int wv, int lane, int tid, double* col,
+                          double* dg, double2* rb) {
+    const int kend = (G + 1 < NJ / REG_G) ? min(m, W * REG_G * (G + 1) + wv - 1) : m;
+    for (; k < kend; k++) {
+        const double* ck = col + (k & 1) * (R * WAVE * REG_G) + (k / W) % REG_G;  // column k, row 0
+        double cv[R];
+        unsigned long long key = 0;
+#pragma unroll
+        for (int h = 0; h < R; h++) {
+            const int r = lane + WAVE * h;
+            cv[h] = (r < m) ? ck[r * REG_G] : 0.0;
+            if (r < m && posr[h] == m) {
+                const unsigned long long kh = pivot_key(cv[h], r);
+                key = kh > key ? kh : key;
+            }
+        }
+        key = wave_umax_uniform(key);
+        const int p = 255 - (int)(key & 0xffull);
+        const int pl = p & (WAVE - 1), ph = p / WAVE;
+        const double piv = readlane_f64((R == 1 || ph == 0) ? cv[0] : cv[R - 1], pl);
+        if (piv == 0.0) return false;  // the same LDS data in every wave: a uniform exit
+#pragma unroll
+        for (int h = 0; h < R; h++)
+            if (lane + WAVE * h == p) posr[h] = k;
+        if (tid == 0) dg[k] = piv;
+        const double inv = 1.0 / piv;
+        double l[R];
+#pragma unroll
+        for (int h = 0; h < R; h++) l[h] = (lane + WAVE * h < m && posr[h] == m) ? cv[h] * inv : 0.0;
+        const bool own = (k + 1) % W == wv;  // column k+1 lies in group G of this wave
+        double* cg = col + ((k + 1) & 1) * (R * WAVE * REG_G);
+        if (R == 1 || ph == 0)
+            reg_step<R, NJ, 0, G>(a, l, own, lane, pl, rb, cg);
+        else
+            reg_step<R, NJ, R - 1, G>(a, l, own, lane, pl, rb, cg);
+        if (W > 1) __syncthreads();
+    }
+    if constexpr (G + 1 < NJ / REG_G)
+        return reg_phase<W, R, NJ, G + 1>(a, posr, k, m, wv, lane, tid, col, dg, rb);
+    else
+        return true;
+}
+
+// DIAG: the variant that also writes the backward error and growth factor (its residual
+// code needs registers the plain variant does without).
+template <int W, int R, int NJ, bool DIAG>
+__global__ void __launch_bounds__(WAVE * W) weights_f64_reg_kernel(WParams P) {
+    static_assert(NJ % REG_G == 0, "slots come in whole update groups");
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ double s_amax[W], s_umax[W];
+    if ((int)blockIdx.x >= P.count) return;
+    const int s = P.order[blockIdx.x];
+    const int tid = threadIdx.x, lane = tid & (WAVE - 1);
+    const int wv = __builtin_amdgcn_readfirstlane(tid / WAVE);  // wave-uniform column owner
+    Sys S;
+    S.P = &P;
+    S.s = s;
+    S.off = P.st_off[s];
+    S.n = P.st_off[s + 1] - S.off;
+    S.m = P.m;
+    S.T = P.T;
+    S.ld = 0;
+    S.eps = P.eps[s];
+    S.e2 = S.eps * S.eps;
+    S.sph = false;
+    for (int t = 0; t < S.T; t++) S.sph |= (P.types[t] == RBFX_LAMBDA || P.types[t] == RBFX_THETA);
+    const int m = S.m, T = S.T, wcols = m + T;
+    double2* rb = (double2*)smem + wv * (NJ / 2);  // this wave's pivot-row buffer
+    double* col = (double*)smem + W * NJ;          // [2][64 R][REG_G]: the group holding column k
+    S.X = col + 2 * R * WAVE * REG_G;
+    S.SPH = S.X + 3 * S.n;
+    S.XS = S.BS = nullptr;
+    S.perm = nullptr;
+    double* dg = S.SPH + 2 * S.n;  // pivots u_kk
+    double* U = dg + m;            // packed upper triangle by columns
+    double* BS = U + (size_t)m * (m + 1) / 2;  // [m][T]: staged R by row, later R' by pivot order
+    for (int j = tid; j < S.n; j += WAVE * W) {
+        const double* q = P.xyz + 3 * (size_t)P.st_idx[S.off + j];
+        S.X[3 * j] = q[0];
+        S.X[3 * j + 1] = q[1];
+        S.X[3 * j + 2] = q[2];
+        if (S.sph) {
+            double lam, th;
+            g_sph(q[0], q[1], q[2], lam, th);
+            S.SPH[2 * j] = lam;
+            S.SPH[2 * j + 1] = th;
+        }
+    }
+    __syncthreads();
+    for (int q = tid; q < m * T; q += WAVE * W) BS[q] = rhs_entry(S, q / T, q - (q / T) * T);
+    __syncthreads();
+    double a[R][NJ];
+    double amax = 0.0;
+#pragma unroll
+    for (int h = 0; h < R; h++) {
+        const int r = lane + WAVE * h;
+#pragma unroll
+        for (int j = 0; j < NJ; j++) {
+            const int c = W * j + wv;
+            double v = 0.0;
+            if (c < m) {
+                if (r < m) {
+                    v = m_entry(r, c, S.n, S.e2, S.X, P.sph);
+                    amax = fmax(amax, fabs(v));
+                }
+            } else if (c < wcols) {
+                if (r < m) v = BS[r * T + (c - m)];
+            }
+            a[h][j] = v;
+            __builtin_amdgcn_sched_barrier(0);  // one entry at a time: short live ranges
+        }
+    }
+    amax = wave_max(amax);
+    if (lane == 0) s_amax[wv] = amax;
+    int posr[R];
+#pragma unroll
+    for (int h = 0; h < R; h++) {
+        posr[h] = m;
+        if (wv == 0) {  // group 0 of wave 0 holds column 0
+            double2* dst = (double2*)(col + (lane + WAVE * h) * REG_G);
+#pragma unroll
+            for (int q = 0; q < REG_G / 2; q++) dst[q] = make_double2(a[h][2 * q], a[h][2 * q + 1]);
+        }
+    }
+    __syncthreads();  // column 0 stored; the staged R has been read
+    int k = 0;
+    const bool ok = reg_phase<W, R, NJ, 0>(a, posr, k, m, wv, lane, tid, col, dg, rb);
+    if (!ok) {
+        if (wv == 0) write_failed(S, P);
+        return;
+    }
+    // U part (columns posr+1 .. m-1) and eliminated right-hand sides of each row, by pivot order
+    double um = 0.0;
+#pragma unroll
+    for (int h = 0; h < R; h++) {
+        const int i = posr[h];
+        if (lane + WAVE * h >= m) continue;
+#pragma unroll
+        for (int j = 0; j < NJ; j++) {
+            const int c = W * j + wv;
+            if (c < m) {
+                if (c >= i) um = fmax(um, fabs(a[h][j]));
+                if (c > i) U[(size_t)c * (c + 1) / 2 + i] = a[h][j];
+            } else if (c < wcols) {
+                BS[i * T + (c - m)] = a[h][j];
+            }
+        }
+    }
+    if (DIAG) {
+        um = wave_max(um);
+        if (lane == 0) s_umax[wv] = um;
+    }
+    __syncthreads();  // U, BS and the pivots complete
+    // back substitution, right-hand sides t = wv + W*q; lane i holds unknown i (pivot order)
+    constexpr int QT = (MAX_T + W - 1) / W;
+    double b[QT][R];
+#pragma unroll
+    for (int h = 0; h < R; h++) {
+        const int i = lane + WAVE * h;
+#pragma unroll
+        for (int q = 0; q < QT; q++) {
+            const int t = wv + W * q;
+            b[q][h] = (t < T && i < m) ? BS[i * T + t] : 0.0;
+        }
+    }
+    for (int k = m - 1; k >= 0; k--) {
+        const double* uc = U + (size_t)k * (k + 1) / 2;
+        const double ukk = dg[k];
+        double ucol[R];
+#pragma unroll
+        for (int h = 0; h < R; h++) ucol[h] = (lane + WAVE * h < k) ? uc[lane + WAVE * h] : 0.0;
+#pragma unroll
+        for (int q = 0; q < QT; q++) {
+            if (wv + W * q >= T) continue;
+            const double bk = readlane_f64((R == 1 || k < WAVE) ? b[q][0] : b[q][R - 1], k & (WAVE - 1));
+            const double x = bk / ukk;
+#pragma unroll
+            for (int h = 0; h < R; h++) {
+                const int i = lane + WAVE * h;
+                if (i < k)
+                    b[q][h] -= ucol[h] * x;
+                else if (i == k)
+                    b[q][h] = x;
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < QT; q++) {
+        const int t = wv + W * q;
+        if (t >= T) continue;
+        const double scale = (P.types[t] == RBFX_HV) ? P.hv_scale : 1.0;
+#pragma unroll
+        for (int h = 0; h < R; h++) {
+            const int i = lane + WAVE * h;
+            if (i < S.n) P.w[(size_t)t * P.nnz + S.off + i] = b[q][h] * scale;
+        }
+    }
+    if (tid == 0) P.fail_flags[s] = 0;
+    if (!DIAG) return;
+    // diagnostics: x by unknown into BS (each wave overwrites only its own columns)
+#pragma unroll
+    for (int q = 0; q < QT; q++) {
+        const int t = wv + W * q;
+        if (t >= T) continue;
+#pragma unroll
+        for (int h = 0; h < R; h++) {
+            const int i = lane + WAVE * h;
+            if (i < m) BS[i * T + t] = b[q][h];
+        }
+    }
+    __syncthreads();
+    if (wv != 0) return;
+    double am = 0.0, umx = 0.0;
+    for (int w = 0; w < W; w++) {
+        am = fmax(am, s_amax[w]);
+        umx = fmax(umx, s_umax[w]);
+    }
+    const double be = residual<double, MAX_T>(S, (double*)nullptr, BS);
+    if (lane == 0) {
+        P.diag[4 * (size_t)s] = be;
+        P.diag[4 * (size_t)s + 1] = umx / am;
+        P.diag[4 * (size_t)s + 2] = 0;
+        P.diag[4 * (size_t)s + 3] = RBFX_PATH_F64;
+    }
+}
+
 // FP32 path: FP32 factorisation of [M | R], then x += LU^-1 (b - M x) with the residual in
 // FP64 (M rebuilt in FP64, b kept in FP64) until the correction stops shrinking or is below
 // FP64 resolution. A stencil that does not converge is flagged (fail_flags = 2) for an FP64
@@ -638,6 +988,46 @@ __global__ void __launch_bounds__(WAVE) weights_f32ir_kernel(WParams P) {
             P.diag[4 * (size_t)s + 3] = RBFX_PATH_F32_IR;
         }
     }
+}
+
+template <int W, int R, int NJ>
+hipError_t launch_reg(int count, int n, hipStream_t st, const WParams& Q) {
+    if (Q.m > R * WAVE || Q.m + Q.T > W * NJ) return hipErrorInvalidValue;
+    const size_t lds = reg_lds_bytes(Q.m, n, Q.T, W * NJ, R * WAVE);
+    if (Q.diag)
+        hipLaunchKernelGGL((weights_f64_reg_kernel<W, R, NJ, true>), dim3(count), dim3(WAVE * W), lds, st, Q);
+    else
+        hipLaunchKernelGGL((weights_f64_reg_kernel<W, R, NJ, false>), dim3(count), dim3(WAVE * W), lds, st, Q);
+    return hipGetLastError();
+}
+
+// Register-resident FP64 solve: waves W, rows per lane R (R = 1 for m <= 64) and column
+// slots NJ (W * NJ >= m + T) by system size.
+hipError_t launch_f64_reg(int m, int T, int count, int n, hipStream_t st, const WParams& Q) {
+    const int cols = m + T;
+    if (m <= WAVE) {
+        if (cols <= 32) return launch_reg<1, 1, 32>(count, n, st, Q);
+        if (cols <= 48) return launch_reg<2, 1, 24>(count, n, st, Q);
+        if (cols <= 64) return launch_reg<2, 1, 32>(count, n, st, Q);
+        return launch_reg<2, 1, 40>(count, n, st, Q);  // m + T <= 80
+    }
+    if (cols <= 96) return launch_reg<4, 2, 24>(count, n, st, Q);
+    if (cols <= 128) return launch_reg<4, 2, 32>(count, n, st, Q);
+    return launch_reg<4, 2, 40>(count, n, st, Q);  // m + T <= 144
+}
+
+// LDS bytes of the layout launch_f64_reg picks.
+size_t f64_reg_lds(int m, int n, int T) {
+    const int cols = m + T;
+    if (m <= WAVE) return reg_lds_bytes(m, n, T, cols <= 32 ? 32 : cols <= 48 ? 48 : cols <= 64 ? 64 : 80, WAVE);
+    return reg_lds_bytes(m, n, T, cols <= 96 ? 96 : cols <= 128 ? 128 : 160, 2 * WAVE);
+}
+
+// RBFX_WEIGHTS_F64=lds (read at each call) selects the one-wave LDS kernel for FP64 solves;
+// it gives the same weights bit for bit.
+bool f64_use_lds() {
+    const char* e = std::getenv("RBFX_WEIGHTS_F64");
+    return e && std::strcmp(e, "lds") == 0;
 }
 
 __global__ void mark_fallback_kernel(const int32_t* order, int count, double* diag) {
@@ -755,9 +1145,17 @@ extern "C" int rbfx_weights(rbfx_ctx* ctx, int32_t n_stencils, const int32_t* d_
             Q.np = num_monomials(effective_degree(poly_deg, n));
             Q.m = n + Q.np;
             Q.fp32 = ir ? 1 : 0;
+            const bool small_t = T <= 4;
+            if (!ir && !f64_use_lds()) {
+                const size_t rl = f64_reg_lds(Q.m, n, T);
+                if (rl + STATIC_LDS_RESERVE > lds_limit)
+                    return rbfx_fail(ctx, RBFX_E_TOO_LARGE, "weights: system %d x %d with %d RHS needs %zu B of LDS", Q.m, Q.m, T, rl);
+                hipError_t le = launch_f64_reg(Q.m, T, c, n, st, Q);
+                if (le != hipSuccess) return rbfx_fail(ctx, RBFX_E_HIP, "weights: launch (m=%d) failed: %s", Q.m, hipGetErrorString(le));
+                continue;
+            }
             size_t lds = ir ? lds_bytes<float>(Q.m, n, T, true) : lds_bytes<double>(Q.m, n, T, false);
             if (lds + STATIC_LDS_RESERVE > lds_limit) return rbfx_fail(ctx, RBFX_E_TOO_LARGE, "weights: system %d x %d with %d RHS needs %zu B of LDS", Q.m, Q.m, T, lds);
-            const bool small_t = T <= 4;
             if (ir && small_t)
                 hipLaunchKernelGGL(weights_f32ir_kernel<4>, dim3(c), dim3(WAVE), lds, st, Q);
             else if (ir)

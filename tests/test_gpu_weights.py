@@ -135,3 +135,47 @@ def test_weights_singular_reported(ctx):
     with pytest.raises(R.RbfxError) as e:
         ctx.weights(off, np.arange(5, dtype=np.int32), pts, 1.0, R.X, dim=3, poly_deg=0)
     assert e.value.code == -5
+
+
+def test_weights_register_kernel_bit_identical(ctx, monkeypatch):
+    """The register-resident FP64 solve (the default) and the one-wave LDS solve give
+    bit-identical weights and diagnostics (the same pivots and the same per-element operation
+    order) over every register layout (systems of 27..121 unknowns, 4 and 5 right-hand
+    sides), and both report an exactly singular large system."""
+    print("case: register vs LDS kernel", flush=True)
+    ALL_TYPES = (R.X | R.Y | R.Z | R.LAPL | R.R | R.HV | R.LAMBDA | R.THETA | R.LSFC | R.XSFC | R.YSFC | R.ZSFC |
+                 R.INTERP)  # 13 right-hand sides: m + T up to 134
+    N = 20000
+    xyz = R.nodes_fibonacci(N, 1e-3, 1)
+    rng = np.random.default_rng(3)
+    ks = (17, 31, 50, 54, 75, 101)
+    sizes = rng.choice(ks, size=900)
+    rows = rng.choice(N, size=900, replace=False)
+    knn = {k: R.knn_kdtree(xyz, k) for k in ks}
+    c = {17: (0.026, 0.08), 31: (0.035, 0.1), 50: (0.044, 0.14), 54: (0.044, 0.14), 75: (0.051, 0.15),
+         101: (0.058, 0.16)}
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int32)
+    idx = np.concatenate([knn[k][r] for k, r in zip(sizes, rows)]).astype(np.int32)
+    eps = np.array([c[k][0] * np.sqrt(N) - c[k][1] for k in sizes])
+    got = {}
+    for kern in ("lds", "reg"):
+        monkeypatch.setenv("RBFX_WEIGHTS_F64", kern)
+        for mask, deg in ((R.X | R.Y | R.Z | R.LAPL, 2), (R.X | R.Y | R.Z | R.LAPL | R.HV, 3), (ALL_TYPES, 3)):
+            got[kern, mask] = ctx.weights(off, idx, xyz, eps, mask, dim=3, poly_deg=deg, hv_k=2, hv_scale=-1e-3,
+                                          diag=True)
+        # 101 distinct nodes in the plane z = 0.5: the monomial z is exactly zero at every
+        # node, so M has a zero column and elimination meets an exact zero pivot
+        plane = np.column_stack([rng.uniform(-0.2, 0.2, (101, 2)), np.full(101, 0.5)])
+        with pytest.raises(R.RbfxError) as e:
+            ctx.weights(np.array([0, 101], np.int32), np.arange(101, dtype=np.int32), plane, 5.0, R.X, dim=3,
+                        poly_deg=2)
+        assert e.value.code == -5
+    for kern, mask in got:
+        if kern != "reg":
+            continue
+        w, dg = got["reg", mask]
+        assert np.isfinite(w).all()
+        assert np.array_equal(w, got["lds", mask][0]), mask
+        assert np.array_equal(dg, got["lds", mask][1]), mask
+        if mask != ALL_TYPES:  # the angular operators are singular at the poles' stencils
+            assert dg[:, 0].max() <= 64 * 121 * U64
