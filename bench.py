@@ -125,6 +125,78 @@ def stream_copy_gbs(ctx, R, nbytes=1 << 30, reps=10):
     return 2 * nbytes / (ms * 1e-3) / 1e9
 
 
+C3_EPS = {17: (0.026, 0.08), 31: (0.035, 0.1), 50: (0.044, 0.14), 75: (0.051, 0.15), 101: (0.058, 0.16)}
+
+
+def bench_weights(ctx, R, N, cpu, nthreads):
+    """Batched weights with uneven stencils: N jittered sphere nodes, each stencil the n nearest
+    with n drawn uniformly from {17, 31, 50, 75, 101} (seed 1), eps = c1 sqrt(N) - c2 per size
+    (the reference's table, n = 75 interpolated), X/Y/Z/LAPL (dim 3), degree 2 and 3; FP64 LU and
+    FP32 LU + FP64 refinement, each one call over all stencils. With cpu, the C++ restatement
+    times a bounded sample of the same stencils on the host threads and checks the GPU weights
+    of that sample (backward error, forward difference against the per-stencil condition)."""
+    t0 = time.time()
+    xyz = R.nodes_fibonacci(N, 1e-3, 0)
+    knn = R.knn_kdtree(xyz, 101)
+    sizes = np.random.default_rng(1).choice([17, 31, 50, 75, 101], size=N)
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int32)
+    mask_cols = np.arange(101)[None, :] < sizes[:, None]
+    idx = knn[mask_cols].astype(np.int32)
+    eps = np.array([C3_EPS[k][0] * np.sqrt(N) - C3_EPS[k][1] for k in (17, 31, 50, 75, 101)])[
+        np.searchsorted([17, 31, 50, 75, 101], sizes)]
+    log(f"weights workload: {N} stencils, {int(off[-1])} nonzeros, setup {time.time() - t0:.1f}s")
+    d_xyz, d_off, d_idx = ctx.to_device(xyz), ctx.to_device(off), ctx.to_device(idx)
+    mask = R.X | R.Y | R.Z | R.LAPL
+    res = {"N": N, "sizes": [17, 31, 50, 75, 101], "operators": "X,Y,Z,LAPL (dim 3)"}
+    outs = {}
+    for deg in (2, 3):
+        for mode_name, mode in (("f64_lu", R.SOLVE_F64_LU), ("f32_lu_f64_ir", R.SOLVE_F32_LU_F64_IR)):
+            ctx.sync()
+            t = time.perf_counter()
+            d_w, d_dg = ctx.weights(d_off, d_idx, d_xyz, eps, mask, dim=3, poly_deg=deg, mode=mode, diag=True,
+                                    device_out=True)
+            ctx.sync()
+            sec = time.perf_counter() - t
+            dg = d_dg.download()
+            paths = dg[:, 3].astype(int)
+            entry = {"s": round(sec, 4), "stencils_per_s": round(N / sec, 1),
+                     "max_backward_error": float(np.nanmax(dg[:, 0])),
+                     "fp64_fallbacks": int((paths == R.lib.PATH_F32_FALLBACK_F64).sum())}
+            outs[(deg, mode_name)] = d_w
+            res[f"deg{deg}_{mode_name}"] = entry
+            log(f"weights deg {deg} {mode_name}: {sec:.3f}s ({N / sec:.3g} stencils/s), {entry}")
+        # FP32 + refinement against the FP64 LU weights (same stencils, all of them)
+        w64 = outs[(deg, "f64_lu")].download()
+        w32 = outs[(deg, "f32_lu_f64_ir")].download()
+        scale = np.abs(w64).max(axis=0)
+        res[f"deg{deg}_ir_vs_f64_max_rel"] = float((np.abs(w32 - w64).max(axis=0) / np.maximum(scale, 1e-300)).max())
+        for key in list(outs):
+            outs[key].free()
+        outs.clear()
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+
+        rng = np.random.default_rng(2)
+        sample = np.sort(rng.choice(N, size=100000, replace=False))
+        s_off = np.concatenate([[0], np.cumsum(sizes[sample])]).astype(np.int32)
+        s_idx = np.concatenate([idx[off[s]:off[s + 1]] for s in sample]).astype(np.int32)
+        t = time.perf_counter()
+        w_cpu, dg_cpu, _ = O.weights(s_off, s_idx, xyz, eps[sample], mask, dim=3, poly_deg=2, diag=True,
+                                     n_threads=nthreads)
+        tc = time.perf_counter() - t
+        w_gpu, _ = ctx.weights(s_off, s_idx, xyz, eps[sample], mask, dim=3, poly_deg=2, diag=True)
+        rel = np.abs(w_gpu - w_cpu).max() / np.abs(w_cpu).max()
+        res["cpu_baseline"] = {"stencils_per_s": round(len(sample) / tc, 1), "cores": nthreads, "kind": "port",
+                               "sample": f"{len(sample)} stencils of the same size mix, degree 2, FP64 LU "
+                                         f"(oracle/oracle.cpp), OpenMP over stencils, {tc:.1f} s",
+                               "gpu_vs_cpu_max_rel_weight_diff": float(rel),
+                               "cpu_max_backward_error": float(dg_cpu[:, 0].max())}
+        res["gpu_over_cpu_deg2_f64"] = round(res["deg2_f64_lu"]["stencils_per_s"] / (len(sample) / tc), 1)
+        log(f"weights CPU baseline: {res['cpu_baseline']}; GPU/CPU {res['gpu_over_cpu_deg2_f64']}x")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -137,6 +209,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=0,
                     help="CPU baseline RK4 steps (0: as many as fit in about 15 s after one probe step)")
     ap.add_argument("--kernel-only", action="store_true", help="set up, then launch only stage kernels (profiling)")
+    ap.add_argument("--no-weights", action="store_true", help="skip the batched uneven-stencil weights section")
     ap.add_argument("--node-order", action="store_true",
                     help="process rows in node order instead of the Morton locality order")
     args = ap.parse_args()
@@ -339,6 +412,14 @@ def main():
     copy_gbs = stream_copy_gbs(ctx, R)
     log(f"plain copy kernel: {copy_gbs:.0f} GB/s")
 
+    # ---- batched weights with uneven stencils (own workload, reported in extra) ----------------
+    host_threads = int(os.environ.get("OMP_NUM_THREADS") or len(os.sched_getaffinity(0)))
+    weights_c3 = None
+    if not args.no_weights:
+        for a in (ell64, ell32, ell_sp64, ell_sp32):  # the RK4 measurements are done: free HBM
+            a.close()
+        weights_c3 = bench_weights(ctx, R, N, not args.no_cpu, host_threads)
+
     # ---- CPU baseline: the oracle's RK4 step on the same workload (all host cores) --------------
     cpu = None
     if not args.no_cpu:
@@ -398,6 +479,7 @@ def main():
             "copy_kernel_GB_s": round(copy_gbs, 1),
             "spmv": spmv,
             "weights_s": round(t_weights, 4),
+            "weights_uneven": weights_c3,
             "tuned_shapes": tuned,
             "device": info,
         },
