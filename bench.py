@@ -132,7 +132,8 @@ def bench_weights(ctx, R, N, cpu, nthreads):
     """Batched weights with uneven stencils: N jittered sphere nodes, each stencil the n nearest
     with n drawn uniformly from {17, 31, 50, 75, 101} (seed 1), eps = c1 sqrt(N) - c2 per size
     (the reference's table, n = 75 interpolated), X/Y/Z/LAPL (dim 3), degree 2 and 3; FP64 LU and
-    FP32 LU + FP64 refinement, each one call over all stencils. With cpu, the C++ restatement
+    FP32 LU + FP64 refinement, each one timed call over all stencils without diagnostics (the
+    production path), then an untimed call with diagnostics for the backward errors. With cpu, the C++ restatement
     times a bounded sample of the same stencils on the host threads and checks the GPU weights
     of that sample (backward error, forward difference against the per-stencil condition)."""
     t0 = time.time()
@@ -153,11 +154,15 @@ def bench_weights(ctx, R, N, cpu, nthreads):
         for mode_name, mode in (("f64_lu", R.SOLVE_F64_LU), ("f32_lu_f64_ir", R.SOLVE_F32_LU_F64_IR)):
             ctx.sync()
             t = time.perf_counter()
-            d_w, d_dg = ctx.weights(d_off, d_idx, d_xyz, eps, mask, dim=3, poly_deg=deg, mode=mode, diag=True,
-                                    device_out=True)
+            d_w, _ = ctx.weights(d_off, d_idx, d_xyz, eps, mask, dim=3, poly_deg=deg, mode=mode, device_out=True)
             ctx.sync()
             sec = time.perf_counter() - t
+            d_w.free()
+            # diagnostics (backward error, path) from a second, untimed call of the same solve
+            d_w, d_dg = ctx.weights(d_off, d_idx, d_xyz, eps, mask, dim=3, poly_deg=deg, mode=mode, diag=True,
+                                    device_out=True)
             dg = d_dg.download()
+            d_dg.free()
             paths = dg[:, 3].astype(int)
             entry = {"s": round(sec, 4), "stencils_per_s": round(N / sec, 1),
                      "max_backward_error": float(np.nanmax(dg[:, 0])),

@@ -6,14 +6,17 @@
 // and calls arma::solve (LAPACK dgesv: LU with partial pivoting). HV columns are scaled by
 // getHVScalar (295-297). Gaussian phi and its derivatives follow rbf_gaussian.h:40-249.
 //
-// GPU design: one 64-lane wave per stencil, no block barriers. The node coordinates of the
-// stencil are staged in LDS, the augmented matrix [M | R] is assembled straight into LDS
-// (one row per lane), factored in place by right-looking elimination with implicit partial
-// pivoting (pivot = largest |a_rk| among the rows not yet used, found by one wave max over
-// packed keys; the row order is kept in a permutation instead of moving rows), and
-// back-substituted. Stencils are grouped by system size so each launch sizes LDS for its
-// class. FP32 mode factors a float copy and refines x with FP64 residuals recomputed from the
-// node coordinates; stencils whose refinement does not converge are re-solved in FP64.
+// GPU design. Stencils are grouped by system size so each launch sizes its layout for its
+// class. Both solvers factor [M | R] by right-looking elimination with implicit partial
+// pivoting (pivot = largest |a_rk| among the rows not yet used, found by a wave max over
+// packed keys; the row order is kept in a permutation instead of moving rows), then
+// back-substitute.
+//  * FP64 (default): the register-resident solver, 1 to 8 waves per stencil with the matrix
+//    in VGPRs (see "register-resident elimination" below).
+//  * The one-wave LDS solver: [M | R] assembled straight into LDS, one row per lane, no block
+//    barriers. It runs the FP32 factorisation of the refinement mode (x refined with FP64
+//    residuals recomputed from the node coordinates; stencils whose refinement does not
+//    converge are re-solved in FP64) and, with RBFX_WEIGHTS_F64=lds, the FP64 mode.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -1002,7 +1005,10 @@ hipError_t launch_reg(int count, int n, hipStream_t st, const WParams& Q) {
 }
 
 // Register-resident FP64 solve: waves W, rows per lane R (R = 1 for m <= 64) and column
-// slots NJ (W * NJ >= m + T) by system size.
+// slots NJ (W * NJ >= m + T) by system size. Measured at 10^6 stencils per class, degree 2,
+// 4 right-hand sides: m = 27 (1,1,32) 4.5 ms; m = 41 (2,1,24) 7.8 ms; m = 60 (2,1,32) 15.6 ms;
+// m = 85 (4,2,24) 57 ms; m = 111 (8,2,16) 129 ms; wider or narrower layouts of each class
+// were slower (fewer waves per SIMD, or more barrier-bound steps).
 hipError_t launch_f64_reg(int m, int T, int count, int n, hipStream_t st, const WParams& Q) {
     const int cols = m + T;
     if (m <= WAVE) {
@@ -1012,15 +1018,15 @@ hipError_t launch_f64_reg(int m, int T, int count, int n, hipStream_t st, const 
         return launch_reg<2, 1, 40>(count, n, st, Q);  // m + T <= 80
     }
     if (cols <= 96) return launch_reg<4, 2, 24>(count, n, st, Q);
-    if (cols <= 128) return launch_reg<4, 2, 32>(count, n, st, Q);
-    return launch_reg<4, 2, 40>(count, n, st, Q);  // m + T <= 144
+    if (cols <= 128) return launch_reg<8, 2, 16>(count, n, st, Q);
+    return launch_reg<8, 2, 24>(count, n, st, Q);  // m + T <= 144
 }
 
 // LDS bytes of the layout launch_f64_reg picks.
 size_t f64_reg_lds(int m, int n, int T) {
     const int cols = m + T;
     if (m <= WAVE) return reg_lds_bytes(m, n, T, cols <= 32 ? 32 : cols <= 48 ? 48 : cols <= 64 ? 64 : 80, WAVE);
-    return reg_lds_bytes(m, n, T, cols <= 96 ? 96 : cols <= 128 ? 128 : 160, 2 * WAVE);
+    return reg_lds_bytes(m, n, T, cols <= 96 ? 96 : cols <= 128 ? 128 : 192, 2 * WAVE);
 }
 
 // RBFX_WEIGHTS_F64=lds (read at each call) selects the one-wave LDS kernel for FP64 solves;
@@ -1028,6 +1034,17 @@ size_t f64_reg_lds(int m, int n, int T) {
 bool f64_use_lds() {
     const char* e = std::getenv("RBFX_WEIGHTS_F64");
     return e && std::strcmp(e, "lds") == 0;
+}
+
+// bad[0] = 1 and bad[1] = the index when some stencil index is outside [0, n_nodes).
+__global__ void check_index_kernel(const int32_t* __restrict__ idx, int nnz, int n_nodes, int* bad) {
+    for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < nnz; q += (long long)gridDim.x * blockDim.x) {
+        const int v = idx[q];
+        if (v < 0 || v >= n_nodes) {
+            atomicOr(bad, 1);
+            bad[1] = v;
+        }
+    }
 }
 
 __global__ void mark_fallback_kernel(const int32_t* order, int count, double* diag) {
@@ -1090,13 +1107,6 @@ extern "C" int rbfx_weights(rbfx_ctx* ctx, int32_t n_stencils, const int32_t* d_
     if (max_n + num_monomials(effective_degree(poly_deg, max_n)) > MAX_M)
         return rbfx_fail(ctx, RBFX_E_TOO_LARGE, "weights: system size %d exceeds %d", max_n + num_monomials(effective_degree(poly_deg, max_n)), MAX_M);
     P.nnz = off[n_stencils];
-    {
-        std::vector<int32_t> idx(P.nnz);
-        RBFX_HIP_CHECK(ctx, hipMemcpyAsync(idx.data(), d_st_idx, sizeof(int32_t) * (size_t)P.nnz, hipMemcpyDeviceToHost, st));
-        RBFX_HIP_CHECK(ctx, hipStreamSynchronize(st));
-        for (int q = 0; q < P.nnz; q++)
-            if (idx[q] < 0 || idx[q] >= n_nodes) return rbfx_fail(ctx, RBFX_E_INVALID, "weights: stencil index %d out of range", idx[q]);
-    }
     // counting sort of stencil ids by size, largest systems first
     std::vector<int> cnt(MAX_M + 1, 0);
     for (int s = 0; s < n_stencils; s++) cnt[off[s + 1] - off[s]]++;
@@ -1114,7 +1124,9 @@ extern "C" int rbfx_weights(rbfx_ctx* ctx, int32_t n_stencils, const int32_t* d_
     int32_t* d_order = nullptr;
     int* d_flags = nullptr;
     hipError_t e;
-    if ((e = hipMalloc(&d_order, sizeof(int32_t) * n_stencils)) != hipSuccess || (e = hipMalloc(&d_flags, sizeof(int) * n_stencils)) != hipSuccess) {
+    // d_flags: one flag per stencil, then [index check flag, an offending index]
+    if ((e = hipMalloc(&d_order, sizeof(int32_t) * n_stencils)) != hipSuccess ||
+        (e = hipMalloc(&d_flags, sizeof(int) * ((size_t)n_stencils + 2))) != hipSuccess) {
         hipFree(d_order);
         return rbfx_fail(ctx, RBFX_E_NOMEM, "weights: %s", hipGetErrorString(e));
     }
@@ -1124,6 +1136,26 @@ extern "C" int rbfx_weights(rbfx_ctx* ctx, int32_t n_stencils, const int32_t* d_
         hipFree(d_order);
         hipFree(d_flags);
     };
+    {  // every stencil index must name a node: checked on the device before any solve reads it
+        int* d_bad = d_flags + n_stencils;
+        int bad[2] = {0, 0};
+        const int blocks = (int)std::min<long long>(4096, ((long long)P.nnz + 255) / 256);
+        if ((e = hipMemsetAsync(d_bad, 0, 2 * sizeof(int), st)) == hipSuccess) {
+            hipLaunchKernelGGL(check_index_kernel, dim3(std::max(blocks, 1)), dim3(256), 0, st, d_st_idx, P.nnz, n_nodes,
+                               d_bad);
+            if ((e = hipGetLastError()) == hipSuccess &&
+                (e = hipMemcpyAsync(bad, d_bad, sizeof(bad), hipMemcpyDeviceToHost, st)) == hipSuccess)
+                e = hipStreamSynchronize(st);
+        }
+        if (e != hipSuccess) {
+            cleanup();
+            return rbfx_fail(ctx, RBFX_E_HIP, "weights: %s", hipGetErrorString(e));
+        }
+        if (bad[0]) {
+            cleanup();
+            return rbfx_fail(ctx, RBFX_E_INVALID, "weights: stencil index %d out of range", bad[1]);
+        }
+    }
     if ((e = hipMemcpyAsync(d_order, order.data(), sizeof(int32_t) * n_stencils, hipMemcpyHostToDevice, st)) != hipSuccess) {
         cleanup();
         return rbfx_fail(ctx, RBFX_E_HIP, "weights: %s", hipGetErrorString(e));

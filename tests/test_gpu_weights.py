@@ -137,6 +137,22 @@ def test_weights_singular_reported(ctx):
     assert e.value.code == -5
 
 
+def test_weights_bad_index_rejected(ctx):
+    """A stencil index outside the node range is an RBFX_E_INVALID error naming the index,
+    raised before any solve runs; a valid call on the same context works afterwards."""
+    print("case: out-of-range stencil index", flush=True)
+    pts = R.nodes_fibonacci(64, 1e-3, 0)
+    off = np.array([0, 5, 10], np.int32)
+    for badv in (64, -1):
+        idx = np.arange(10, dtype=np.int32)
+        idx[7] = badv
+        with pytest.raises(R.RbfxError) as e:
+            ctx.weights(off, idx, pts, 1.0, R.X, dim=3, poly_deg=0)
+        assert e.value.code == -1 and str(badv) in str(e.value)
+    w, _ = ctx.weights(off, np.arange(10, dtype=np.int32), pts, 1.0, R.X, dim=3, poly_deg=0)
+    assert np.isfinite(w).all()
+
+
 def test_weights_register_kernel_bit_identical(ctx, monkeypatch):
     """The register-resident FP64 solve (the default) and the one-wave LDS solve give
     bit-identical weights and diagnostics (the same pivots and the same per-element operation
