@@ -98,17 +98,27 @@ STAGE_PROFILE = os.path.join(ROOT, "profiles", "rk4_stage_N1M_n50_ops4_F16.md")
 def profile_traffic(kernel):
     """HBM bytes per launch of `kernel` measured by rocprofv3 FETCH_SIZE / WRITE_SIZE passes over
     this same workload (scaled by the copy-kernel calibration), read from the committed profile
-    summary's traffic table; None when the profile holds no row for this kernel instance."""
+    summary's traffic table: (bytes, profiled kernel). The exact instance is preferred; else the
+    same kernel and element types with another tuned launch shape (the autotuner may pick
+    another shape from run to run; the bytes moved differ little between shapes). (None, None)
+    when the profile holds no such row."""
+    rows = {}
     try:
         with open(STAGE_PROFILE) as fh:
             for line in fh:
                 cells = [c.strip() for c in line.split("|")]
                 # traffic table rows: kernel | fetched | written | traffic | algorithmic | ratio | us | GB/s
-                if len(cells) == 10 and cells[1] == f"`{kernel}`":
-                    return float(cells[4])
+                if len(cells) == 10 and cells[1].startswith("`") and cells[4][:1].isdigit():
+                    rows[cells[1].strip("`")] = float(cells[4])
     except OSError:
-        pass
-    return None
+        return None, None
+    if kernel in rows:
+        return rows[kernel], kernel
+    family = kernel.split(",")[:3]
+    for name, b in rows.items():
+        if name.split(",")[:3] == family:
+            return b, name
+    return None, None
 
 
 def stream_copy_gbs(ctx, R, nbytes=1 << 30, reps=10):
@@ -128,7 +138,7 @@ def stream_copy_gbs(ctx, R, nbytes=1 << 30, reps=10):
 C3_EPS = {17: (0.026, 0.08), 31: (0.035, 0.1), 50: (0.044, 0.14), 75: (0.051, 0.15), 101: (0.058, 0.16)}
 
 
-def bench_weights(ctx, R, N, cpu, nthreads):
+def bench_weights(ctx, R, N, cpu, nthreads, degrees=(2, 3), modes=("f64_lu", "f32_lu_f64_ir")):
     """Batched weights with uneven stencils: N jittered sphere nodes, each stencil the n nearest
     with n drawn uniformly from {17, 31, 50, 75, 101} (seed 1), eps = c1 sqrt(N) - c2 per size
     (the reference's table, n = 75 interpolated), X/Y/Z/LAPL (dim 3), degree 2 and 3; FP64 LU and
@@ -150,8 +160,9 @@ def bench_weights(ctx, R, N, cpu, nthreads):
     mask = R.X | R.Y | R.Z | R.LAPL
     res = {"N": N, "sizes": [17, 31, 50, 75, 101], "operators": "X,Y,Z,LAPL (dim 3)"}
     outs = {}
-    for deg in (2, 3):
-        for mode_name, mode in (("f64_lu", R.SOLVE_F64_LU), ("f32_lu_f64_ir", R.SOLVE_F32_LU_F64_IR)):
+    all_modes = {"f64_lu": R.SOLVE_F64_LU, "f32_lu_f64_ir": R.SOLVE_F32_LU_F64_IR}
+    for deg in degrees:
+        for mode_name, mode in ((k, all_modes[k]) for k in modes):
             ctx.sync()
             t = time.perf_counter()
             d_w, _ = ctx.weights(d_off, d_idx, d_xyz, eps, mask, dim=3, poly_deg=deg, mode=mode, device_out=True)
@@ -170,11 +181,12 @@ def bench_weights(ctx, R, N, cpu, nthreads):
             outs[(deg, mode_name)] = d_w
             res[f"deg{deg}_{mode_name}"] = entry
             log(f"weights deg {deg} {mode_name}: {sec:.3f}s ({N / sec:.3g} stencils/s), {entry}")
-        # FP32 + refinement against the FP64 LU weights (same stencils, all of them)
-        w64 = outs[(deg, "f64_lu")].download()
-        w32 = outs[(deg, "f32_lu_f64_ir")].download()
-        scale = np.abs(w64).max(axis=0)
-        res[f"deg{deg}_ir_vs_f64_max_rel"] = float((np.abs(w32 - w64).max(axis=0) / np.maximum(scale, 1e-300)).max())
+        if len(modes) == 2:  # FP32 + refinement against the FP64 LU weights (all stencils)
+            w64 = outs[(deg, "f64_lu")].download()
+            w32 = outs[(deg, "f32_lu_f64_ir")].download()
+            scale = np.abs(w64).max(axis=0)
+            res[f"deg{deg}_ir_vs_f64_max_rel"] = float((np.abs(w32 - w64).max(axis=0) /
+                                                        np.maximum(scale, 1e-300)).max())
         for key in list(outs):
             outs[key].free()
         outs.clear()
@@ -215,6 +227,10 @@ def main():
                     help="CPU baseline RK4 steps (0: as many as fit in about 15 s after one probe step)")
     ap.add_argument("--kernel-only", action="store_true", help="set up, then launch only stage kernels (profiling)")
     ap.add_argument("--no-weights", action="store_true", help="skip the batched uneven-stencil weights section")
+    ap.add_argument("--weights-only", action="store_true",
+                    help="run only the batched uneven-stencil weights section (profiling)")
+    ap.add_argument("--weights-f64-deg2", action="store_true",
+                    help="with --weights-only: only the FP64 LU solve at degree 2 (one launch per size class)")
     ap.add_argument("--node-order", action="store_true",
                     help="process rows in node order instead of the Morton locality order")
     args = ap.parse_args()
@@ -226,6 +242,12 @@ def main():
     import rbffd_gpu_amd as R
 
     N, n, F = args.n_nodes, args.width, args.fields
+    if args.weights_only:
+        ctx = R.Context(0)
+        sel = dict(degrees=(2,), modes=("f64_lu",)) if args.weights_f64_deg2 else {}
+        res = bench_weights(ctx, R, N, False, 1, **sel)
+        log("weights-only done", json.dumps(res))
+        return
     t0 = time.time()
     xyz = R.nodes_fibonacci(N, 1e-3, 0)
     xyz /= np.linalg.norm(xyz, axis=1)[:, None]
@@ -396,7 +418,7 @@ def main():
         kname32 = f"rk4_stage_lds_kernel<float, float, float, 4, {t32['fpt']}, {t32['batch']}, {sg32}, 2>"
     else:
         kname32 = f"rk4_stage_kernel<float, float, float, 4, {t32['fpt']}, {t32['batch']}, {sg32}, {pd32}>"
-    traffic32 = profile_traffic(kname32)
+    traffic32, traffic_kernel = profile_traffic(kname32)
     log(f"stage kernel avg: fp32 {avg_ms32 * 1e3:.1f} us ({ach32:.0f} GB/s), "
         f"fp64 {np.mean(stage_ms['fp64']) * 1e3:.1f} us ({ach64:.0f} GB/s)")
 
@@ -467,7 +489,8 @@ def main():
                      "frac": round(ach32 / HBM_PEAK_GBS, 4), "traffic": traffic32,
                      "kernel": kname32, "algorithmic_bytes_per_launch": int(avg_b32),
                      "avg_launch_us": round(avg_ms32 * 1e3, 2),
-                     "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE, profiles/rk4_stage_N1M_n50_ops4_F16.md"},
+                     "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE, profiles/rk4_stage_N1M_n50_ops4_F16.md",
+                     "traffic_kernel": traffic_kernel},
         "cpu_baseline": cpu,
         "extra": {
             "fp64_steps_per_s": round(K / wall64, 2),
