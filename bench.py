@@ -233,6 +233,9 @@ def main():
                     help="with --weights-only: only the FP64 LU solve at degree 2 (one launch per size class)")
     ap.add_argument("--node-order", action="store_true",
                     help="process rows in node order instead of the Morton locality order")
+    ap.add_argument("--spiral-numbering", action="store_true",
+                    help="keep the generator's (golden-angle spiral) node numbering instead of numbering the "
+                         "nodes along the Morton curve")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only: the row-partitioned multi-GPU "
@@ -251,6 +254,10 @@ def main():
     t0 = time.time()
     xyz = R.nodes_fibonacci(N, 1e-3, 0)
     xyz /= np.linalg.norm(xyz, axis=1)[:, None]
+    if not args.spiral_numbering:
+        # number the nodes along a Morton curve (the same point set, relabelled): the 64-byte field
+        # records of a stencil's nodes then share cache lines and pages
+        xyz = np.ascontiguousarray(xyz[R.order_morton(xyz)])
     st = R.knn_kdtree(xyz, n)
     log(f"nodes + kd-tree stencils: {time.time() - t0:.1f}s")
     ctx = R.Context(0)

@@ -3,9 +3,11 @@
 // angle alpha, 12 days per revolution, RK4, optional Delta^k hyperviscosity scaled
 // -gamma N^-k. Nodes: icosahedral level L (N = 10*4^L + 2) or the seeded spiral.
 // Prints the reference's norm lines, then one JSON line with the errors and timings.
+// The nodes are renumbered along a Morton curve before the stencils are built (numbering
+// "morton", the default) or kept in generator order ("input").
 //
 // usage: rbfx_cosine_bell <ico:L|fib:N> <n> <eps_c1> <eps_c2> <hv_k> <hv_gamma> <steps_per_rev>
-//                         <revolutions> <f64|f32> [alpha_denom=2] [dim=2]
+//                         <revolutions> <f64|f32> [alpha_denom=2] [dim=2] [numbering=morton]
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -21,7 +23,9 @@ using namespace rbfx;
 
 int main(int argc, char** argv) {
     if (argc < 10) {
-        std::fprintf(stderr, "usage: %s ico:L|fib:N n c1 c2 hv_k hv_gamma steps_per_rev revs f64|f32 [alpha_denom] [dim]\n",
+        std::fprintf(stderr,
+                     "usage: %s ico:L|fib:N n c1 c2 hv_k hv_gamma steps_per_rev revs f64|f32 [alpha_denom] [dim] "
+                     "[morton|input]\n",
                      argv[0]);
         return 2;
     }
@@ -49,11 +53,13 @@ int main(int argc, char** argv) {
     bool f32 = std::string(argv[9]) == "f32";
     double alpha_denom = argc > 10 ? std::atof(argv[10]) : 2.0;
     int dim = argc > 11 ? std::atoi(argv[11]) : 2;
+    const bool morton = argc > 12 ? std::string(argv[12]) != "input" : true;
     const double T = 1036800.0, a = 1.0;
     double alpha = alpha_denom != 0 ? M_PI / alpha_denom : 0.0;
 
     auto t0 = std::chrono::steady_clock::now();
     Domain grid{Grid(nodes)};
+    if (morton) grid.sortNodesMorton();
     grid.generateStencils((unsigned)n, Grid::ST_KDTREE);
     auto t1 = std::chrono::steady_clock::now();
     RBFFD_HIP der(RBFFD::LAMBDA | RBFFD::THETA | RBFFD::HV, &grid, dim, 0);

@@ -24,6 +24,26 @@ void Grid::checkStencilSize() {
     }
 }
 
+std::vector<int32_t> Grid::sortNodesMorton() {
+    const size_t n = node_list.size();
+    std::vector<double> xyz(3 * n);
+    for (size_t i = 0; i < n; i++)
+        for (int d = 0; d < 3; d++) xyz[3 * i + d] = node_list[i][d];
+    std::vector<int32_t> perm(n);
+    if (n && rbfx_order_morton((int32_t)n, xyz.data(), perm.data()) != RBFX_OK) {
+        std::cout << "[Grid] ERROR: rbfx_order_morton failed" << std::endl;
+        std::exit(EXIT_FAILURE);
+    }
+    std::vector<NodeType> sorted(n);
+    for (size_t i = 0; i < n; i++) sorted[i] = node_list[perm[i]];
+    node_list.swap(sorted);
+    stencil_map.clear();
+    avg_stencil_radii.clear();
+    min_stencil_radii.clear();
+    max_stencil_radii.clear();
+    return perm;
+}
+
 void Grid::generateStencils(unsigned int st_max_size, st_generator_t generator_choice) {
     max_st_size = st_max_size;
     generateStencils(generator_choice);

@@ -63,12 +63,15 @@ def test_cosine_bell_driver_matches_cpu(tmp_path):
     (eps = 0.035 sqrt(N) - 0.1, HV k = 4, gamma = 5e-2, dim 2), one revolution in 500 RK4 steps
     through CosineBell_HIP. The run is replayed on the CPU (restated CosineBell::solve +
     advanceRungeKutta4) with the weights the GPU computed: the solutions agree to 1e-12; the
-    GPU weights agree with the CPU-solved weights to the conditioning bound."""
+    GPU weights agree with the CPU-solved weights to the conditioning bound. The driver numbers
+    the nodes along the Morton curve (Grid::sortNodesMorton); the replay applies the same
+    permutation."""
     out = run([os.path.join(BIN, "rbfx_cosine_bell"), "ico:4", "31", "0.035", "0.1", "4", "0.05", "500", "1", "f64"],
               tmp_path)
     res = json.loads(out.strip().splitlines()[-1])
     u_gpu = np.fromfile(tmp_path / "cosine_bell_solution.bin", dtype=np.float64)
     xyz = R.nodes_icosahedral(4)
+    xyz = np.ascontiguousarray(xyz[R.order_morton(xyz)])
     N = xyz.shape[0]
     n = 31
     w_gpu = np.fromfile(tmp_path / "cosine_bell_weights.bin", dtype=np.float64).reshape(3, N, n)
