@@ -296,9 +296,10 @@ def main():
             e.set_row_order(order)
         log(f"Morton row order: {time.time() - t0:.2f}s")
         t0 = time.time()
-        # node cap per tile: 28 KB of staged records per block (F fields of 4 or 8 bytes)
-        for name, e, fb in (("fp64", ell64, 8), ("fp32", ell32, 4)):
-            tl, k_max, n_tiles = e.build_local(64, (28 * 1024) // (F * fb))
+        # node cap per tile: 28 KB (FP32) / 36 KB (FP64) of staged records per block, the
+        # fastest caps of a sweep over 16-36 KB and 32- or 64-row tiles
+        for name, e, fb, cap in (("fp64", ell64, 8, 36 * 1024), ("fp32", ell32, 4, 28 * 1024)):
+            tl, k_max, n_tiles = e.build_local(64, cap // (F * fb))
             log(f"LDS staging tables {name}: {n_tiles} tiles of <= {tl} rows, at most {k_max} distinct nodes per tile")
         log(f"LDS staging tables: {time.time() - t0:.2f}s")
     t0 = time.time()

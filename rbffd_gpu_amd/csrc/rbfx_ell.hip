@@ -150,6 +150,12 @@ __global__ void ell_order_check_kernel(const int* __restrict__ count, int n, int
     if (r < n && count[r] != 1) atomicAdd(bad, 1);
 }
 
+// moved collects the positions whose row is not the position itself (0: the identity order).
+__global__ void ell_order_moved_kernel(const int32_t* __restrict__ order, int n, int* __restrict__ moved) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n && order[p] != p) atomicAdd(moved, 1);
+}
+
 // old_pos[row] = layout position of row in the current layout.
 __global__ void ell_old_pos_kernel(const int32_t* __restrict__ old_order, int n, int32_t* __restrict__ old_pos) {
     int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -416,7 +422,7 @@ int rbfx_ell_set_row_order(rbfx_ell* e, const int32_t* d_order) {
         hipFree(idx_n);
         hipFree(w_n);
     };
-    if ((err = hipMalloc(&d_count, sizeof(int) * ((size_t)n + 1))) != hipSuccess ||
+    if ((err = hipMalloc(&d_count, sizeof(int) * ((size_t)n + 2))) != hipSuccess ||
         (err = hipMalloc(&d_old_pos, sizeof(int32_t) * (size_t)n)) != hipSuccess ||
         (err = hipMalloc(&d_new_order, sizeof(int32_t) * (size_t)n)) != hipSuccess ||
         (err = hipMalloc(&idx_n, sizeof(int32_t) * entries)) != hipSuccess ||
@@ -424,17 +430,20 @@ int rbfx_ell_set_row_order(rbfx_ell* e, const int32_t* d_order) {
         release();
         return rbfx_fail(ctx, RBFX_E_NOMEM, "ell_set_row_order: allocation failed: %s", hipGetErrorString(err));
     }
-    // validate: a permutation of 0..n-1 (count[n] collects out-of-range entries and repeats)
-    int bad = 0;
+    // validate: a permutation of 0..n-1 (count[n] collects out-of-range entries and repeats;
+    // count[n+1] the positions that move, 0 for the identity)
+    int flags[2] = {0, 0};
     const unsigned g = (unsigned)((n + 255) / 256);
-    hipMemsetAsync(d_count, 0, sizeof(int) * ((size_t)n + 1), s);
+    hipMemsetAsync(d_count, 0, sizeof(int) * ((size_t)n + 2), s);
     ell_order_count_kernel<<<g, 256, 0, s>>>(d_order, n, d_count, d_count + n);
     ell_order_check_kernel<<<g, 256, 0, s>>>(d_count, n, d_count + n);
-    hipMemcpyAsync(&bad, d_count + n, sizeof(int), hipMemcpyDeviceToHost, s);
+    ell_order_moved_kernel<<<g, 256, 0, s>>>(d_order, n, d_count + n + 1);
+    hipMemcpyAsync(flags, d_count + n, sizeof(flags), hipMemcpyDeviceToHost, s);
     if ((err = hipStreamSynchronize(s)) != hipSuccess) {
         release();
         return rbfx_fail(ctx, RBFX_E_HIP, "ell_set_row_order: %s", hipGetErrorString(err));
     }
+    const int bad = flags[0];
     if (bad) {
         release();
         return rbfx_fail(ctx, RBFX_E_INVALID, "ell_set_row_order: not a permutation of the %d rows", n);
@@ -458,6 +467,10 @@ int rbfx_ell_set_row_order(rbfx_ell* e, const int32_t* d_order) {
     e->idx_t = idx_n;
     e->w_t = w_n;
     e->order = d_new_order;
+    if (flags[1] == 0) {  // the identity: node order, which the kernels take without an order array
+        hipFree(e->order);
+        e->order = nullptr;
+    }
     free_local(e);  // the local tables describe the previous layout
     e->layout_version++;
     idx_n = nullptr;
