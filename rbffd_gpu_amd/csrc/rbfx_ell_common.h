@@ -425,9 +425,29 @@ __device__ __forceinline__ void ell_stage_tile(const UT* __restrict__ u, int F, 
                                                UT* lds) {
     typedef unsigned v4u __attribute__((ext_vector_type(4)));
     const int C = F * (int)sizeof(UT) / 16;
-    for (int q = threadIdx.x; q < K * C; q += blockDim.x) {
-        const int k = q / C, c = q - k * C;
-        reinterpret_cast<v4u*>(lds)[q] = reinterpret_cast<const v4u*>(u + (size_t)nodes[k] * F)[c];
+    const int total = K * C, B = blockDim.x;
+    // Rounds of up to SI chunks per thread: all node ids of a round are requested, then all
+    // records, then the LDS writes, so a round costs two memory latencies however many chunks
+    // it holds (a plain loop waits twice per chunk).
+    constexpr int SI = 8;
+    for (int q0 = threadIdx.x; q0 < total; q0 += SI * B) {
+        int kk[SI];
+        v4u rec[SI];
+#pragma unroll
+        for (int i = 0; i < SI; i++) {
+            const int q = q0 + i * B;
+            kk[i] = q < total ? nodes[q / C] : 0;
+        }
+#pragma unroll
+        for (int i = 0; i < SI; i++) {
+            const int q = q0 + i * B;
+            if (q < total) rec[i] = reinterpret_cast<const v4u*>(u + (size_t)kk[i] * F)[q - (q / C) * C];
+        }
+#pragma unroll
+        for (int i = 0; i < SI; i++) {
+            const int q = q0 + i * B;
+            if (q < total) reinterpret_cast<v4u*>(lds)[q] = rec[i];
+        }
     }
 }
 
