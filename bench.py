@@ -422,8 +422,9 @@ def main():
     t32 = tuned["fp32"]
     sg32 = F // t32["fpt"] if t32.get("share") else 0
     pd32 = t32["share"] if t32.get("share", 0) >= 2 else 0
-    if t32.get("share") == 4:  # LDS-staged kernel (prefetch depth 2)
-        kname32 = f"rk4_stage_lds_kernel<float, float, float, 4, {t32['fpt']}, {t32['batch']}, {sg32}, 2>"
+    if t32.get("share") in (4, 5):  # LDS-staged kernel (prefetch depth 2; 5: matrix-core multiply-adds)
+        mx = "true" if t32["share"] == 5 else "false"
+        kname32 = f"rk4_stage_lds_kernel<float, float, float, 4, {t32['fpt']}, {t32['batch']}, {sg32}, 2, {mx}>"
     else:
         kname32 = f"rk4_stage_kernel<float, float, float, 4, {t32['fpt']}, {t32['batch']}, {sg32}, {pd32}>"
     traffic32, traffic_kernel = profile_traffic(kname32)

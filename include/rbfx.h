@@ -228,7 +228,10 @@ int rbfx_rk4_stage(rbfx_rk4* rk, int32_t stage, double dt);
  * them inside the group, instead of every lane loading them. share = 2 or 3: the same, with
  * those loads issued 2 or 3 batches ahead and the next batch's gathers issued before the
  * current batch is consumed (3-4 operators; fpt 4 with 4 lanes or fpt 8 with 2 lanes, and
- * for depth 2 also fpt 4 with 2 lanes or fpt 2 with 4 lanes). */
+ * for depth 2 also fpt 4 with 2 lanes or fpt 2 with 4 lanes). share = 4: LDS-staged gathers
+ * (needs rbfx_ell_build_local; the block is the tile). share = 5: share 4 with the FP32
+ * multiply-adds on the matrix cores (FP32 weights, fields and accumulation, fpt 4 with 4
+ * lanes, i.e. 16 fields); bit-identical to share 4. */
 int rbfx_tune_set(rbfx_ell* ell, int32_t kind, int32_t n_fields, int32_t field_type, int32_t accum, int32_t fpt,
                   int32_t block, int32_t batch, int32_t share);
 int rbfx_tune_get(rbfx_ell* ell, int32_t kind, int32_t n_fields, int32_t field_type, int32_t accum, int32_t* fpt,

@@ -98,6 +98,46 @@ __device__ __forceinline__ void store_pack(T* __restrict__ p, const T (&in)[N]) 
     }
 }
 
+// Streaming (non-temporal) forms of load_pack / store_pack for data touched once per kernel:
+// 16-byte chunks with the nt hint when N * sizeof(T) is a multiple of 16, plain otherwise.
+template <typename T, int N>
+__device__ __forceinline__ void load_pack_nt(const T* __restrict__ p, T (&out)[N]) {
+    constexpr int B = (int)(N * sizeof(T));
+    if constexpr (B % 16 == 0) {
+        typedef unsigned v4u __attribute__((ext_vector_type(4)));
+        constexpr int C = 16 / sizeof(T);
+#pragma unroll
+        for (int c = 0; c < N / C; c++) {
+            v4u q = __builtin_nontemporal_load(reinterpret_cast<const v4u*>(p + c * C));
+            Pack<T, C> r;
+            __builtin_memcpy(&r, &q, 16);
+#pragma unroll
+            for (int i = 0; i < C; i++) out[c * C + i] = r.v[i];
+        }
+    } else {
+        load_pack<T, N>(p, out);
+    }
+}
+template <typename T, int N>
+__device__ __forceinline__ void store_pack_nt(T* __restrict__ p, const T (&in)[N]) {
+    constexpr int B = (int)(N * sizeof(T));
+    if constexpr (B % 16 == 0) {
+        typedef unsigned v4u __attribute__((ext_vector_type(4)));
+        constexpr int C = 16 / sizeof(T);
+#pragma unroll
+        for (int c = 0; c < N / C; c++) {
+            Pack<T, C> r;
+#pragma unroll
+            for (int i = 0; i < C; i++) r.v[i] = in[c * C + i];
+            v4u q;
+            __builtin_memcpy(&q, &r, 16);
+            __builtin_nontemporal_store(q, reinterpret_cast<v4u*>(p + c * C));
+        }
+    } else {
+        store_pack<T, N>(p, in);
+    }
+}
+
 // Weights of all NOPS operators at one (slot,row): NOPS == 3 is loaded as 3 scalars.
 template <typename WT, int NOPS>
 __device__ __forceinline__ void load_w(const WT* __restrict__ p, WT (&w)[NOPS]) {
@@ -110,8 +150,8 @@ __device__ __forceinline__ void load_w(const WT* __restrict__ p, WT (&w)[NOPS]) 
 
 // Weights of one (slot,row). FP32 weights carry the streaming (non-temporal) hint: they are
 // read once per stage and should not displace the gathered field records from the caches
-// (measured: 3 % faster stages); for FP64 weights the hint measured 13 % slower, so they
-// load normally.
+// (measured: 3 % faster stages); for FP64 weights the hint measured 13 % slower (and again
+// 14 % slower with the LDS-staged stage kernel), so they load normally.
 template <typename WT, int NOPS>
 __device__ __forceinline__ void load_w_nt(const WT* __restrict__ p, WT (&w)[NOPS]) {
     typedef float f4 __attribute__((ext_vector_type(4)));
@@ -527,6 +567,86 @@ __device__ __forceinline__ void ell_gather_dot_lds(const uint16_t* __restrict__ 
             }
         });
     }
+}
+
+// FP32 variant of ell_gather_dot_lds for 16 fields on 4 lanes per row (FPT = 4, G = 4) whose
+// multiply-adds run on the matrix cores: one v_mfma_f32_4x4x1_16b_f32 treats the 16 rows of a
+// wave as 16 independent 4x4 blocks and adds, per block, the outer product of a 4-vector held
+// one value per lane of the row's group (A) with another (B). Per slot k and field offset g:
+//   A at lane q = w_q[row][k]                     (operator q; 0 for q >= NOPS)
+//   B at lane q = u[idx_k][4q + g]                (the lane's own 16-byte record fragment)
+//   D_g[o] at lane q += A(lane o) * B(lane q)     = acc[o][g] of the lane, field 4q + g.
+// Each lane loads its operator's weight of every slot (the 4 lanes of a row read 16 adjacent
+// bytes), so no weight broadcast is needed; the local node ids are still loaded once per
+// group and broadcast (DPP). The k order per accumulator is the same as in the VALU cores.
+// MFMA reads operands across the lanes of a block only, and a row's 4 lanes are active or
+// inactive together, so rows that exited early do not disturb the others.
+template <int NOPS, int UB, int PD>
+__device__ __forceinline__ void ell_gather_dot_lds_mfma(const uint16_t* __restrict__ loc_idx,
+                                                        const float* __restrict__ w_t, const float* lds, int p,
+                                                        int width, int fofs, int q, float (&acc)[NOPS][4]) {
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    constexpr int G = 4, L = UB / G, F = 16;
+    static_assert(UB % G == 0 && NOPS <= 4, "batch of whole groups, at most 4 operators");
+    f4 D[4];
+#pragma unroll
+    for (int g = 0; g < 4; g++) D[g] = f4{0.f, 0.f, 0.f, 0.f};
+    const size_t base = ell_row_base(p, width);
+    const uint16_t* lip = loc_idx + base;
+    const bool has_op = q < NOPS;
+    const float* wq = w_t + base * NOPS + (has_op ? q : 0);
+    const size_t wstride = (size_t)ELL_TILE * NOPS;
+    const float* lb = lds + fofs;
+    const int nbatch = (width + UB - 1) / UB;
+    int nbl[PD][L];
+    float wl[PD][UB];
+    auto own = [&](int jb, int (&nb)[L], float (&w)[UB]) {
+#pragma unroll
+        for (int l = 0; l < L; l++) {
+            const int j = jb + l * G + q;
+            nb[l] = (int)__builtin_nontemporal_load(lip + (size_t)(j < width ? j : width - 1) * ELL_TILE);
+        }
+        if (jb + UB <= width) {  // whole batch inside the row (uniform)
+#pragma unroll
+            for (int k = 0; k < UB; k++) {
+                const float v = __builtin_nontemporal_load(wq + (size_t)(jb + k) * wstride);
+                w[k] = (NOPS == 4 || has_op) ? v : 0.f;
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < UB; k++) {
+                const int j = jb + k;
+                const float v = __builtin_nontemporal_load(wq + (size_t)(j < width ? j : width - 1) * wstride);
+                w[k] = (j < width && (NOPS == 4 || has_op)) ? v : 0.f;
+            }
+        }
+    };
+    StaticFor<0, PD>::run([&](auto S) { own(decltype(S)::value * UB, nbl[decltype(S)::value], wl[decltype(S)::value]); });
+    for (int b0 = 0; b0 < nbatch; b0 += PD) {
+        StaticFor<0, PD>::run([&](auto S) {
+            constexpr int s = decltype(S)::value;
+            const int b = b0 + s;
+            if (b < nbatch) {
+                float v[UB][4];
+#pragma unroll
+                for (int l = 0; l < L; l++)
+                    StaticFor<0, G>::run([&](auto S2) {
+                        constexpr int s2 = decltype(S2)::value;
+                        const int li = bcast_group<G, s2>(nbl[s][l]);
+                        load_pack<float, 4>(lb + li * F, v[l * G + s2]);
+                    });
+#pragma unroll
+                for (int k = 0; k < UB; k++)
+#pragma unroll
+                    for (int g = 0; g < 4; g++) D[g] = __builtin_amdgcn_mfma_f32_4x4x1f32(wl[s][k], v[k][g], D[g], 0, 0, 0);
+                own(b * UB + PD * UB, nbl[s], wl[s]);
+            }
+        });
+    }
+#pragma unroll
+    for (int o = 0; o < NOPS; o++)
+#pragma unroll
+        for (int g = 0; g < 4; g++) acc[o][g] = D[g][o];
 }
 
 // Maps a hardware block index to a logical tile so that each of the 8 XCDs (which receive

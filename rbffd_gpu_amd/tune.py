@@ -22,16 +22,20 @@ def _pipe_ok(n_ops, fpt, G, share):
                            (share == 2 and ((fpt == 4 and G == 2) or (fpt == 2 and G == 4))))
 
 
-def candidates(n_fields, kind=KIND_SPMM, n_ops=1, local=None, field_bytes=8):
+def candidates(n_fields, kind=KIND_SPMM, n_ops=1, local=None, field_bytes=8, fp32_accum=False):
     """(fpt, block, batch, share) shapes the library accepts for n_fields fields.
     local = (tile_rows, k_max) of an operator set with LDS-staging tables: adds the staged
-    shapes (share 4; the block is the tile) when a tile's records fit in LDS_BYTES."""
+    shapes (share 4; the block is the tile) when a tile's records fit in LDS_BYTES, and with
+    FP32 fields and accumulation (fp32_accum) the staged shapes whose multiply-adds run on
+    the matrix cores (share 5, 16 fields on 4 lanes)."""
     if kind == KIND_RK4 and local and local[0] > 0 and n_ops >= 3 and (n_fields * field_bytes) % 16 == 0 \
             and local[1] * n_fields * field_bytes <= LDS_BYTES:
         for fpt, G in ((4, 4), (8, 2)):
             if n_fields == fpt * G and local[0] * G <= 256:
                 for batch in (4, 8):
                     yield fpt, local[0] * G, batch, 4
+                    if fp32_accum and field_bytes == 4 and fpt == 4:
+                        yield fpt, local[0] * G, batch, 5
     for fpt in (1, 2, 4, 8):
         if n_fields % fpt:
             continue
@@ -77,7 +81,8 @@ def tune_rk4(ctx, rk: RK4, ell: EllOperators, field_type, accum, dt=1e-6, reps=3
     F = rk.n_fields
     results = []
     local = ell.local_info()
-    for fpt, block, batch, share in candidates(F, KIND_RK4, ell.n_ops, local, 8 if field_type == 0 else 4):
+    fp32 = field_type == 1 and accum == 1 and ell.storage != 0
+    for fpt, block, batch, share in candidates(F, KIND_RK4, ell.n_ops, local, 8 if field_type == 0 else 4, fp32):
         ell.tune_set(KIND_RK4, F, field_type, accum, fpt, block, batch, share)
         for s in (1, 2, 3, 4):
             rk.stage(s, dt)  # warm

@@ -128,6 +128,44 @@ def test_rk4_amplification_factor(ctx):
     ell.close()
 
 
+@pytest.mark.parametrize("n_ops", [3, 4])
+def test_rk4_matrix_core_fp32_bit_identical(ctx, c1, n_ops):
+    """FP32 weights, fields and accumulation: the LDS-staged stage kernel whose multiply-adds
+    run on the matrix cores (4x4x1 FP32 MFMA, one 4x4 block per row) takes the slots in the
+    same order with one rounding per multiply-add, so its states are bit-identical to the
+    VALU kernel's; with 3 operators the fourth lane of each row feeds zeros."""
+    print(f"case: rk4 matrix-core fp32 n_ops={n_ops}", flush=True)
+    xyz, st, ws = _c1_ops(c1)
+    ws = ws[:n_ops]
+    F = 16
+    u0 = _fields(xyz, F, seed=13)
+    coefs = [xyz[:, 1], -xyz[:, 0], 0.1 * xyz[:, 2], None][:n_ops]
+    ell = R.EllOperators(ctx, st, ws, storage=R.F32)
+    ell.set_row_order(R.order_morton(xyz))
+    tl, k_max, n_tiles = ell.build_local(64, 128)  # variable tile sizes, partly filled blocks
+    rk = R.RK4(ctx, ell, F, R.F32, R.F32, coefs=coefs)
+    from rbffd_gpu_amd import tune as TU
+    shapes = [c for c in TU.candidates(F, TU.KIND_RK4, n_ops, (tl, k_max, n_tiles), 4, True) if c[3] in (4, 5)]
+    assert sorted({c[3] for c in shapes}) == [4, 5]
+    outs = {}
+    for fpt, block, batch, share in shapes:
+        ell.tune_set(TU.KIND_RK4, F, R.F32, R.F32, fpt, block, batch, share)
+        rk.set_state(u0)
+        rk.step(1e-3, 3)
+        outs[(fpt, batch, share)] = rk.get_state()
+    base = outs[(4, 8, 4)]
+    for k, v in outs.items():
+        print(f"  shape {k}: max |diff| {np.abs(v - base).max():.3e}")
+    for k, v in outs.items():
+        assert np.array_equal(v, base), k
+    # the matrix-core shapes exist only for FP32 accumulation
+    with pytest.raises(R.RbfxError):
+        ell.tune_set(TU.KIND_RK4, F, R.F32, R.F64, 4, tl * 4, 8, 5)
+    R.RBFX.rbfx_tune_clear()
+    rk.close()
+    ell.close()
+
+
 def test_rk4_fp32_storage_deviation(ctx, c1):
     """FP32-stored weights and stage fields with FP64 state: deviation from the FP64 path."""
     xyz, st, ws = _c1_ops(c1)
