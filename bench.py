@@ -262,11 +262,25 @@ def main():
         # number the nodes along a Morton curve (the same point set, relabelled): the 64-byte field
         # records of a stencil's nodes then share cache lines and pages
         xyz = np.ascontiguousarray(xyz[R.order_morton(xyz)])
+    t_nodes = time.time() - t0
+    t0 = time.time()
     st = R.knn_kdtree(xyz, n)
-    log(f"nodes + kd-tree stencils: {time.time() - t0:.1f}s")
+    t_kdtree = time.time() - t0
+    log(f"nodes {t_nodes:.1f}s, kd-tree stencils (host, all threads) {t_kdtree:.2f}s")
     ctx = R.Context(0)
     info = ctx.info()
     log("device", info)
+    # the same stencils from the GPU search (rows with distance ties from the host kd-tree)
+    R.knn_gpu(ctx, xyz[:1000].copy(), n)  # warm the kernel
+    t0 = time.time()
+    st_gpu, knn_host_rows = R.knn_gpu(ctx, xyz, n)
+    t_knn_gpu = time.time() - t0
+    knn = {"n_nodes": N, "k": n, "kdtree_host_s": round(t_kdtree, 3), "gpu_s": round(t_knn_gpu, 3),
+           "host_rows": knn_host_rows, "identical_to_kdtree": bool(np.array_equal(st_gpu, st)),
+           "host_threads": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count()))}
+    del st_gpu
+    log(f"GPU kNN stencils: {t_knn_gpu:.3f}s ({knn_host_rows} rows from the host kd-tree), "
+        f"identical to the kd-tree: {knn['identical_to_kdtree']}")
 
     # ---- weights on the GPU ----------------------------------------------------------------
     mask = R.XSFC | R.YSFC | R.ZSFC | R.HV  # columns in ascending bit order: HV, XSFC, YSFC, ZSFC
@@ -543,6 +557,7 @@ def main():
             "spmv": spmv,
             "weights_s": round(t_weights, 4),
             "weights_uneven": weights_c3,
+            "knn": knn,
             "tuned_shapes": tuned,
             "device": info,
         },

@@ -104,6 +104,14 @@ int rbfx_timer_stop(rbfx_ctx* ctx, double* ms);
  * (j = 0 is node i itself), bit-identical to the reference, ties included.
  * Host memory, CPU work; n_threads <= 0 uses all OpenMP threads. */
 int rbfx_knn_kdtree(int32_t n_nodes, const double* xyz, int32_t k, int32_t* out_idx, int32_t n_threads);
+/* The same stencils from a GPU search (replaces the query loop of Grid::generateStencilsKDTree,
+ * src/grids/grid_interface.cpp:688-725; host arrays in and out). A query whose k+1 nearest
+ * distances are all distinct has one answer, the k nearest in ascending distance, which the
+ * GPU computes exactly over a uniform grid of cells with the kd-tree's distance arithmetic.
+ * Queries with a distance tie among their k+1 nearest (the kd-tree's heap order decides
+ * those) are recomputed on the host by the kd-tree; n_host_rows (may be NULL) receives how
+ * many. The result is bit-identical to rbfx_knn_kdtree. k <= n_nodes. */
+int rbfx_knn_gpu(rbfx_ctx* ctx, int32_t n_nodes, const double* xyz, int32_t k, int32_t* out_idx, int32_t* n_host_rows);
 /* Seeded node set on the unit sphere: golden-angle (Fibonacci) spiral plus an independent
  * uniform +-jitter per coordinate (SplitMix64 counter stream), not re-projected.
  * Stands in for GridReader::readNodeList's input files (src/grids/grid_reader.cpp:36-100). */

@@ -17,6 +17,7 @@ from .lib import (  # noqa: F401
     F64, F32, F32_RN,
     SOLVE_F64_LU, SOLVE_F32_LU_F64_IR, POLY_SPHERE,
     knn_kdtree,
+    knn_gpu,
     nodes_fibonacci,
     nodes_icosahedral,
     order_morton,

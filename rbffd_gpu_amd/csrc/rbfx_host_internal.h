@@ -17,3 +17,8 @@
 int rbfx_build_local_tables(const int32_t* idx_layout, int n_rows, int n_pad, int width, int tile_layout, int lt,
                             int cap, std::vector<int32_t>& pos, std::vector<int32_t>& off, std::vector<int32_t>& nodes,
                             std::vector<uint16_t>& loc_idx);
+
+// kd-tree k-nearest-neighbour rows (same answers as rbfx_knn_kdtree) for the query rows
+// rows[0..n_rows) only, written at out_idx[row * k ..]; k <= n_nodes, rows in range.
+extern "C" __attribute__((visibility("hidden"))) int rbfx_knn_kdtree_rows(int32_t n_nodes, const double* xyz, int32_t k, const int32_t* rows,
+                                    int32_t n_rows, int32_t* out_idx);

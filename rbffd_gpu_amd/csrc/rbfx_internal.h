@@ -10,6 +10,19 @@
 
 #include "rbfx.h"
 
+// FP64 add and multiply that round on their own. hipcc contracts a * b + c into a fused
+// multiply-add by default, also across inlined calls (this HIP version's __dadd_rn and
+// __dmul_rn are plain + and *); operations written under contract(off) carry no contraction
+// permission, so these two stay separate wherever they are inlined.
+__device__ __forceinline__ double dadd_nc(double a, double b) {
+#pragma clang fp contract(off)
+    return a + b;
+}
+__device__ __forceinline__ double dmul_nc(double a, double b) {
+#pragma clang fp contract(off)
+    return a * b;
+}
+
 struct rbfx_ctx {
     int device = 0;
     hipStream_t stream = nullptr;

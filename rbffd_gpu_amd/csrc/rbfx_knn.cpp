@@ -347,6 +347,28 @@ int rbfx_knn_kdtree(int32_t n_nodes, const double* xyz, int32_t k, int32_t* out_
     return RBFX_OK;
 }
 
+// kd-tree answers for the listed query rows only (out_idx keeps the n_nodes x k layout; other
+// rows are left untouched). Used for the rows that the GPU search hands back.
+int rbfx_knn_kdtree_rows(int32_t n_nodes, const double* xyz, int32_t k, const int32_t* rows, int32_t n_rows,
+                         int32_t* out_idx) {
+    if (n_rows <= 0) return RBFX_OK;
+    KdTree3 tree(xyz, n_nodes);
+#ifdef _OPENMP
+#pragma omp parallel
+#endif
+    {
+        MaxHeapD pq;
+#ifdef _OPENMP
+#pragma omp for schedule(static, 64)
+#endif
+        for (int32_t r = 0; r < n_rows; r++) {
+            const int32_t i = rows[r];
+            tree.knn(xyz + 3 * (size_t)i, k, pq, out_idx + (size_t)i * k);
+        }
+    }
+    return RBFX_OK;
+}
+
 // Icosahedral node set: the 12 vertices of an icosahedron, each face split into four
 // `level` times with every new edge midpoint projected onto the unit sphere;
 // N = 10 * 4^level + 2 (level 7: 163842). Vertex order: the 12 corners, then midpoints in

@@ -13,8 +13,8 @@
 //   stage 2: acc = acc + 2 f  X' = u + (0.5 dt) f
 //   stage 3: acc = acc + 2 f  X' = u + dt f
 //   stage 4: u = u + (dt/6) (acc + f)       [X' = u in storage precision]
-// The additions and products are kept unfused (__dadd_rn/__dmul_rn) so they round exactly
-// where the reference's loops round. State u and acc are FP64; X (the gathered stage field)
+// The additions and products are kept unfused (dadd_nc / dmul_nc, written under
+// `#pragma clang fp contract(off)`) so they round exactly where the reference's loops round. State u and acc are FP64; X (the gathered stage field)
 // is FP64 or FP32 storage. In the FP64 path stage 1 gathers u itself.
 #include <cstring>
 
@@ -133,7 +133,7 @@ __device__ __forceinline__ void stage_update(const StageArgs& a, int row, int fo
 #pragma unroll
         for (int o = 0; o < NOPS; o++) {
             double dv = (double)d[o][f];
-            s = a.coef[o] ? __dadd_rn(s, __dmul_rn(c[o], dv)) : __dadd_rn(s, dv);
+            s = a.coef[o] ? dadd_nc(s, dmul_nc(c[o], dv)) : dadd_nc(s, dv);
         }
         fv[f] = s;
     }
@@ -143,7 +143,7 @@ __device__ __forceinline__ void stage_update(const StageArgs& a, int row, int fo
     double xo[FPT];
     if (a.stage == 1) {
 #pragma unroll
-        for (int f = 0; f < FPT; f++) xo[f] = __dadd_rn(un[f], __dmul_rn(a.hdt, fv[f]));
+        for (int f = 0; f < FPT; f++) xo[f] = dadd_nc(un[f], dmul_nc(a.hdt, fv[f]));
         store_pack_nt<double, FPT>(a.acc + e, fv);
     } else if (a.stage < 4) {
         double ac[FPT];
@@ -151,15 +151,15 @@ __device__ __forceinline__ void stage_update(const StageArgs& a, int row, int fo
         double h = (a.stage == 2) ? a.hdt : a.dt;
 #pragma unroll
         for (int f = 0; f < FPT; f++) {
-            ac[f] = __dadd_rn(ac[f], 2.0 * fv[f]);
-            xo[f] = __dadd_rn(un[f], __dmul_rn(h, fv[f]));
+            ac[f] = dadd_nc(ac[f], 2.0 * fv[f]);
+            xo[f] = dadd_nc(un[f], dmul_nc(h, fv[f]));
         }
         store_pack_nt<double, FPT>(a.acc + e, ac);
     } else {
         double ac[FPT];
         load_pack_nt<double, FPT>(a.acc + e, ac);
 #pragma unroll
-        for (int f = 0; f < FPT; f++) xo[f] = __dadd_rn(un[f], __dmul_rn(a.dt6, __dadd_rn(ac[f], fv[f])));
+        for (int f = 0; f < FPT; f++) xo[f] = dadd_nc(un[f], dmul_nc(a.dt6, dadd_nc(ac[f], fv[f])));
         store_pack_nt<double, FPT>(a.u + e, xo);
     }
     if (a.xout) {

@@ -67,6 +67,7 @@ def load_library(path=None):
         "rbfx_timer_start": (C.c_int, [vp]),
         "rbfx_timer_stop": (C.c_int, [vp, C.POINTER(f64)]),
         "rbfx_knn_kdtree": (C.c_int, [i32, vp, i32, vp, i32]),
+        "rbfx_knn_gpu": (C.c_int, [vp, i32, vp, i32, vp, vp]),
         "rbfx_nodes_fibonacci": (C.c_int, [i32, f64, u64, vp]),
         "rbfx_nodes_icosahedral": (C.c_int, [i32, vp, C.POINTER(i32)]),
         "rbfx_order_morton": (C.c_int, [i32, vp, vp]),
@@ -126,6 +127,17 @@ def knn_kdtree(xyz, k, n_threads=0):
     if rc:
         raise RbfxError(rc, "knn_kdtree failed")
     return out
+
+
+def knn_gpu(ctx, xyz, k):
+    """The kd-tree stencils of knn_kdtree computed by the GPU search (bit-identical; rows with
+    distance ties are recomputed by the host kd-tree). Returns (int32 (N, k), host rows)."""
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    n = xyz.shape[0]
+    out = np.empty((n, k), dtype=np.int32)
+    n_host = C.c_int32()
+    ctx.check(RBFX.rbfx_knn_gpu(ctx.h, n, _ptr(xyz), k, _ptr(out), C.byref(n_host)), "knn_gpu")
+    return out, n_host.value
 
 
 def nodes_fibonacci(n, jitter=1e-3, seed=0):

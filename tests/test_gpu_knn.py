@@ -1,0 +1,59 @@
+"""GPU k-nearest-neighbour stencils against the host kd-tree (itself bit-identical to the
+reference's external/kdtree search, tests/test_knn.py): index lists must match bit for bit.
+
+Node sets cover the cases that decide the split between the GPU search and the host
+fallback: jittered spiral nodes (distinct distances almost everywhere), the icosahedral grid
+(symmetric, many exactly equal distances), a volume and a plane, duplicated nodes (distance
+ties at 0), and k close to N.
+"""
+import numpy as np
+import pytest
+
+import rbffd_gpu_amd as R
+
+pytestmark = pytest.mark.gpu
+
+
+def _ico(level):
+    import ctypes as C
+    n = C.c_int32()
+    R.RBFX.rbfx_nodes_icosahedral(level, None, C.byref(n))
+    xyz = np.empty((n.value, 3))
+    assert R.RBFX.rbfx_nodes_icosahedral(level, xyz.ctypes.data_as(C.c_void_p), C.byref(n)) == 0
+    return xyz
+
+
+def _sets():
+    rng = np.random.default_rng(4)
+    plane = rng.random((3000, 3))
+    plane[:, 2] = 0.5
+    dup = R.nodes_fibonacci(2000, 1e-3, 2)
+    dup[1500:1520] = dup[100:120]  # 20 duplicated nodes: ties at distance 0
+    return [("fibonacci 20000", R.nodes_fibonacci(20000, 1e-3, 0), (31, 50)),
+            ("icosahedral level 4", _ico(4), (31, 50)),
+            ("cube 5000", rng.random((5000, 3)), (17, 50)),
+            ("plane 3000", plane, (31,)),
+            ("duplicates 2000", dup, (17,)),
+            ("small 60", R.nodes_fibonacci(60, 1e-3, 1), (59, 60))]
+
+
+@pytest.mark.parametrize("case", range(6))
+def test_knn_gpu_matches_kdtree(ctx, case):
+    name, xyz, ks = _sets()[case]
+    for k in ks:
+        print(f"case: knn_gpu {name} k={k}", flush=True)
+        ref = R.knn_kdtree(xyz, k)
+        got, n_host = R.knn_gpu(ctx, xyz, k)
+        print(f"  {n_host} of {xyz.shape[0]} rows from the host kd-tree")
+        assert np.array_equal(got, ref)
+        if not name.startswith("duplicates"):
+            assert np.array_equal(got[:, 0], np.arange(xyz.shape[0]))  # each row starts with its node
+        if name.startswith("fibonacci"):
+            assert n_host < xyz.shape[0] // 100  # distinct distances: the GPU answers nearly every row
+        if name.startswith("duplicates"):
+            assert n_host >= 40  # both copies of each duplicated node tie at distance 0
+
+
+def test_knn_gpu_rejects_k_above_n(ctx):
+    with pytest.raises(R.RbfxError):
+        R.knn_gpu(ctx, R.nodes_fibonacci(10, 1e-3, 0), 11)
