@@ -25,7 +25,7 @@ int main(int argc, char** argv) {
     if (argc < 10) {
         std::fprintf(stderr,
                      "usage: %s ico:L|fib:N n c1 c2 hv_k hv_gamma steps_per_rev revs f64|f32 [alpha_denom] [dim] "
-                     "[morton|input]\n",
+                     "[morton|input] [kdtree|gpu]\n",
                      argv[0]);
         return 2;
     }
@@ -54,13 +54,15 @@ int main(int argc, char** argv) {
     double alpha_denom = argc > 10 ? std::atof(argv[10]) : 2.0;
     int dim = argc > 11 ? std::atoi(argv[11]) : 2;
     const bool morton = argc > 12 ? std::string(argv[12]) != "input" : true;
+    const Grid::st_generator_t stencils =
+        argc > 13 && std::string(argv[13]) == "gpu" ? Grid::ST_KDTREE_GPU : Grid::ST_KDTREE;
     const double T = 1036800.0, a = 1.0;
     double alpha = alpha_denom != 0 ? M_PI / alpha_denom : 0.0;
 
     auto t0 = std::chrono::steady_clock::now();
     Domain grid{Grid(nodes)};
     if (morton) grid.sortNodesMorton();
-    grid.generateStencils((unsigned)n, Grid::ST_KDTREE);
+    grid.generateStencils((unsigned)n, stencils);
     auto t1 = std::chrono::steady_clock::now();
     RBFFD_HIP der(RBFFD::LAMBDA | RBFFD::THETA | RBFFD::HV, &grid, dim, 0);
     int useHV = hv_gamma != 0.0;

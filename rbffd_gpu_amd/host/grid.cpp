@@ -50,8 +50,9 @@ void Grid::generateStencils(unsigned int st_max_size, st_generator_t generator_c
 }
 
 void Grid::generateStencils(st_generator_t generator_choice) {
-    if (generator_choice != ST_KDTREE) {
-        std::cout << "[Grid] ERROR: only the kd-tree stencil generator (ST_KDTREE) is provided" << std::endl;
+    if (generator_choice != ST_KDTREE && generator_choice != ST_KDTREE_GPU) {
+        std::cout << "[Grid] ERROR: only the kd-tree stencil generator (ST_KDTREE, ST_KDTREE_GPU) is provided"
+                  << std::endl;
         std::exit(EXIT_FAILURE);
     }
     checkStencilSize();
@@ -60,7 +61,18 @@ void Grid::generateStencils(st_generator_t generator_choice) {
     for (size_t i = 0; i < n; i++)
         for (int d = 0; d < 3; d++) xyz[3 * i + d] = node_list[i][d];
     std::vector<int32_t> st(n * (size_t)max_st_size);
-    int rc = rbfx_knn_kdtree((int32_t)n, xyz.data(), (int32_t)max_st_size, st.data(), 0);
+    int rc;
+    if (generator_choice == ST_KDTREE_GPU) {
+        rbfx_ctx* ctx = nullptr;
+        rc = rbfx_create(0, &ctx);
+        if (rc == RBFX_OK) {
+            rc = rbfx_knn_gpu(ctx, (int32_t)n, xyz.data(), (int32_t)max_st_size, st.data(), nullptr);
+            if (rc != RBFX_OK) std::cout << "[Grid] " << rbfx_last_error(ctx) << std::endl;
+            rbfx_destroy(ctx);
+        }
+    } else {
+        rc = rbfx_knn_kdtree((int32_t)n, xyz.data(), (int32_t)max_st_size, st.data(), 0);
+    }
     if (rc != RBFX_OK) {
         std::cout << "[Grid] ERROR: kd-tree stencil generation failed (" << rc << ")" << std::endl;
         std::exit(EXIT_FAILURE);

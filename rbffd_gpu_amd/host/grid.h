@@ -26,7 +26,9 @@ typedef std::vector<int> StencilType;
 
 class Grid {
   public:
-    enum st_generator_t { ST_BRUTE_FORCE = 0, ST_KDTREE, ST_HASH };
+    // ST_KDTREE_GPU (not in the reference): the kd-tree's lists from the GPU search
+    // rbfx_knn_gpu, bit-identical to ST_KDTREE; it uses device 0 for the call.
+    enum st_generator_t { ST_BRUTE_FORCE = 0, ST_KDTREE, ST_HASH, ST_KDTREE_GPU };
 
     Grid() {}
     explicit Grid(unsigned int num_nodes) : nb_nodes(num_nodes), global_num_nodes(num_nodes) {}
@@ -35,8 +37,9 @@ class Grid {
     virtual ~Grid() {}
 
     // Grid::generateStencils (grid_interface.cpp:507-536): ST_KDTREE runs the reference's
-    // kd-tree algorithm (bit-identical lists) through rbfx_knn_kdtree. The brute-force and
-    // hash generators of the reference are not part of this build and exit with an error.
+    // kd-tree algorithm (bit-identical lists) through rbfx_knn_kdtree; ST_KDTREE_GPU gives the
+    // same lists through rbfx_knn_gpu. The brute-force and hash generators of the reference
+    // are not part of this build and exit with an error.
     void generateStencils(st_generator_t generator_choice = ST_KDTREE);
     void generateStencils(unsigned int st_max_size, st_generator_t generator_choice = ST_KDTREE);
     void computeStencilRadii();  // grid_interface.cpp:539-633 (min/max/avg distance to centre)

@@ -90,3 +90,17 @@ def test_cosine_bell_driver_matches_cpu(tmp_path):
         rel = np.abs(w_gpu[t].reshape(-1) - w_ref[t]).max() / np.abs(w_ref[t]).max()
         assert rel < 1e-3, (t, rel)
     print("cosine bell after one revolution:", {k: res[k] for k in ("l2rel", "linfrel")})
+
+
+def test_cosine_bell_driver_gpu_stencils_identical(tmp_path):
+    """Grid::generateStencils(ST_KDTREE_GPU) gives the kd-tree's lists, so the whole cosine-bell
+    run (weights, 500 RK4 steps) is bit-identical to the one with ST_KDTREE stencils."""
+    sols = []
+    for gen in ("kdtree", "gpu"):
+        d = tmp_path / gen
+        d.mkdir()
+        run([os.path.join(BIN, "rbfx_cosine_bell"), "ico:4", "31", "0.035", "0.1", "4", "0.05", "500", "1", "f64",
+             "2", "2", "morton", gen], d)
+        sols.append(np.fromfile(d / "cosine_bell_solution.bin", dtype=np.float64))
+    assert np.array_equal(sols[0], sols[1])
+
