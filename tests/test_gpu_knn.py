@@ -57,3 +57,37 @@ def test_knn_gpu_matches_kdtree(ctx, case):
 def test_knn_gpu_rejects_k_above_n(ctx):
     with pytest.raises(R.RbfxError):
         R.knn_gpu(ctx, R.nodes_fibonacci(10, 1e-3, 0), 11)
+
+
+def test_knn_gpu_reference_golden_lists(ctx, c1):
+    """Against the reference's own kd-tree output (fixtures from oracle/_ref/ref_knn): the C1
+    node set and the tie-heavy grid, lattice and duplicate sets, bit for bit."""
+    import os
+    from conftest import GOLDEN
+    got, _ = R.knn_gpu(ctx, c1["xyz"], 31)
+    assert np.array_equal(got, c1["st"])
+    g = np.load(os.path.join(GOLDEN, "ties_knn.npz"))
+    for name, ks in (("grid", (5, 9, 13, 31)), ("lattice", (7, 27, 50)), ("dups", (9, 17))):
+        for k in ks:
+            got, n_host = R.knn_gpu(ctx, g[name], k)
+            print(f"  {name} k={k}: {n_host} of {g[name].shape[0]} rows from the host kd-tree")
+            assert np.array_equal(got, g[f"{name}_k{k}"]), (name, k)
+
+
+def test_knn_gpu_large_sets_digest(ctx):
+    """N = 163842 with k = 101 (32 queries per block) and N = 10^6 with k = 50: SHA-256 of the
+    GPU stencil array equals that of the reference's kd-tree output."""
+    import hashlib
+    import json
+    import os
+    from conftest import GOLDEN
+    with open(os.path.join(GOLDEN, "large_knn.json")) as f:
+        big = json.load(f)
+    for key, rec in big.items():
+        print(f"case: knn_gpu digest {key}", flush=True)
+        xyz = R.nodes_fibonacci(rec["n"], rec["jitter"], rec["seed"])
+        assert hashlib.sha256(xyz.tobytes()).hexdigest() == rec["xyz_sha256"], "node generator changed"
+        st, n_host = R.knn_gpu(ctx, xyz, rec["k"])
+        print(f"  {n_host} of {rec['n']} rows from the host kd-tree")
+        assert np.array_equal(st[:16], np.array(rec["first_rows"], np.int32))
+        assert hashlib.sha256(st.tobytes()).hexdigest() == rec["st_sha256"], key
