@@ -124,6 +124,18 @@ def profile_traffic(kernel):
     return None, None
 
 
+def cpu_model():
+    """Host CPU model name (first 'model name' line of /proc/cpuinfo), or None."""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def stream_copy_gbs(ctx, R, nbytes=1 << 30, reps=10):
     """Measured HBM bandwidth of a plain copy kernel (read + write bytes / time)."""
     a = ctx.alloc(nbytes // 4, np.float32)
@@ -212,6 +224,14 @@ def bench_weights(ctx, R, N, cpu, nthreads, degrees=(2, 3), modes=("f64_lu", "f3
                                          f"(oracle/oracle.cpp), OpenMP over stencils, {tc:.1f} s",
                                "gpu_vs_cpu_max_rel_weight_diff": float(rel),
                                "cpu_max_backward_error": float(dg_cpu[:, 0].max())}
+        # the same solves on one host thread, on the first 10^4 stencils of the sample
+        s1 = 10000
+        t = time.perf_counter()
+        O.weights(s_off[:s1 + 1], s_idx[:s_off[s1]], xyz, eps[sample[:s1]], mask, dim=3, poly_deg=2, n_threads=1)
+        t1 = time.perf_counter() - t
+        res["cpu_baseline"]["one_thread"] = {"stencils_per_s": round(s1 / t1, 1), "sample": f"{s1} stencils, {t1:.2f} s"}
+        res["cpu_baseline"]["cpu_model"] = cpu_model()
+        res["cpu_baseline"]["nproc"] = os.cpu_count()
         res["gpu_over_cpu_deg2_f64"] = round(res["deg2_f64_lu"]["stencils_per_s"] / (len(sample) / tc), 1)
         log(f"weights CPU baseline: {res['cpu_baseline']}; GPU/CPU {res['gpu_over_cpu_deg2_f64']}x")
     return res
@@ -513,7 +533,11 @@ def main():
                                           time_budget=15.0)
         cpu = {"value": round(cpu_steps / tc, 4), "unit": "RK4 steps/s", "cores": nthreads, "kind": "port",
                "sample": f"{cpu_steps} FP64 RK4 steps ({tc:.1f} s) of the same N={N}, n={n}, 4-operator, "
-                         f"{F}-field workload with the C++ restatement (oracle/oracle.cpp), OpenMP over rows"}
+                         f"{F}-field workload with the C++ restatement (oracle/oracle.cpp), OpenMP over rows",
+               "cpu_model": cpu_model(), "nproc": os.cpu_count()}
+        if args.cpu_steps <= 0:  # one step on one host thread, for the per-core figure
+            _, s1, t1 = O.rk4_step(st, w_host, coefs_h, u0, dt, n_steps=1, n_threads=1, time_budget=1e30)
+            cpu["one_thread"] = {"value": round(s1 / t1, 4), "sample": f"{s1} FP64 RK4 step, {t1:.1f} s"}
         log(f"CPU baseline: {cpu}")
 
     value = K / wall32
