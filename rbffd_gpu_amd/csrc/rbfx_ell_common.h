@@ -573,7 +573,7 @@ __device__ __forceinline__ void ell_gather_dot_lds(const uint16_t* __restrict__ 
 // multiply-adds run on the matrix cores: one v_mfma_f32_4x4x1_16b_f32 treats the 16 rows of a
 // wave as 16 independent 4x4 blocks and adds, per block, the outer product of a 4-vector held
 // one value per lane of the row's group (A) with another (B). Per slot k and field offset g:
-//   A at lane q = w_q[row][k]                     (operator q; 0 for q >= NOPS)
+//   A at lane q = w_q[row][k]                     (operator q)
 //   B at lane q = u[idx_k][4q + g]                (the lane's own 16-byte record fragment)
 //   D_g[o] at lane q += A(lane o) * B(lane q)     = acc[o][g] of the lane, field 4q + g.
 // Each lane loads its operator's weight of every slot (the 4 lanes of a row read 16 adjacent
@@ -593,56 +593,62 @@ __device__ __forceinline__ void ell_gather_dot_lds_mfma(const uint16_t* __restri
     for (int g = 0; g < 4; g++) D[g] = f4{0.f, 0.f, 0.f, 0.f};
     const size_t base = ell_row_base(p, width);
     const uint16_t* lip = loc_idx + base;
-    const bool has_op = q < NOPS;
-    const float* wq = w_t + base * NOPS + (has_op ? q : 0);
+    // lanes q >= NOPS load operator 0's weights: their A values only reach row q of each 4x4
+    // block, i.e. the accumulators of operator q, which are not returned
+    const float* wq = w_t + base * NOPS + (q < NOPS ? q : 0);
     const size_t wstride = (size_t)ELL_TILE * NOPS;
     const float* lb = lds + fofs;
     const int nbatch = (width + UB - 1) / UB;
     int nbl[PD][L];
     float wl[PD][UB];
+    // Loads of the batch starting at slot jb. Slots past the row's width read the clamped last
+    // slot (in range); the consumer masks them. No branch and no use of the loaded values
+    // here, so PD batches of loads stay in flight.
     auto own = [&](int jb, int (&nb)[L], float (&w)[UB]) {
 #pragma unroll
         for (int l = 0; l < L; l++) {
             const int j = jb + l * G + q;
-            nb[l] = (int)__builtin_nontemporal_load(lip + (size_t)(j < width ? j : width - 1) * ELL_TILE);
+            nb[l] = (int)__builtin_nontemporal_load(lip + (size_t)min(j, width - 1) * ELL_TILE);
         }
-        if (jb + UB <= width) {  // whole batch inside the row (uniform)
 #pragma unroll
-            for (int k = 0; k < UB; k++) {
-                const float v = __builtin_nontemporal_load(wq + (size_t)(jb + k) * wstride);
-                w[k] = (NOPS == 4 || has_op) ? v : 0.f;
-            }
-        } else {
+        for (int k = 0; k < UB; k++) w[k] = __builtin_nontemporal_load(wq + (size_t)min(jb + k, width - 1) * wstride);
+    };
+    // MASK: slots at or past the width get weight 0 (0 * a finite record: the accumulators do
+    // not change)
+    auto consume = [&](int (&nb)[L], float (&w)[UB], bool mask, int jb) {
+        float v[UB][4];
 #pragma unroll
-            for (int k = 0; k < UB; k++) {
-                const int j = jb + k;
-                const float v = __builtin_nontemporal_load(wq + (size_t)(j < width ? j : width - 1) * wstride);
-                w[k] = (j < width && (NOPS == 4 || has_op)) ? v : 0.f;
-            }
+        for (int l = 0; l < L; l++)
+            StaticFor<0, G>::run([&](auto S2) {
+                constexpr int s2 = decltype(S2)::value;
+                const int li = bcast_group<G, s2>(nb[l]);
+                load_pack<float, 4>(lb + li * F, v[l * G + s2]);
+            });
+#pragma unroll
+        for (int k = 0; k < UB; k++) {
+            const float a = mask ? w[k] * (jb + k < width ? 1.f : 0.f) : w[k];
+#pragma unroll
+            for (int g = 0; g < 4; g++) D[g] = __builtin_amdgcn_mfma_f32_4x4x1f32(a, v[k][g], D[g], 0, 0, 0);
         }
     };
     StaticFor<0, PD>::run([&](auto S) { own(decltype(S)::value * UB, nbl[decltype(S)::value], wl[decltype(S)::value]); });
-    for (int b0 = 0; b0 < nbatch; b0 += PD) {
+    // the pipelined loop takes whole batches only, a multiple of PD of them; the rest (at most
+    // PD - 1 whole batches and the partial one) are already loaded when it ends
+    const int nfull = (width / UB) / PD * PD;
+    for (int b0 = 0; b0 < nfull; b0 += PD) {
         StaticFor<0, PD>::run([&](auto S) {
             constexpr int s = decltype(S)::value;
-            const int b = b0 + s;
-            if (b < nbatch) {
-                float v[UB][4];
-#pragma unroll
-                for (int l = 0; l < L; l++)
-                    StaticFor<0, G>::run([&](auto S2) {
-                        constexpr int s2 = decltype(S2)::value;
-                        const int li = bcast_group<G, s2>(nbl[s][l]);
-                        load_pack<float, 4>(lb + li * F, v[l * G + s2]);
-                    });
-#pragma unroll
-                for (int k = 0; k < UB; k++)
-#pragma unroll
-                    for (int g = 0; g < 4; g++) D[g] = __builtin_amdgcn_mfma_f32_4x4x1f32(wl[s][k], v[k][g], D[g], 0, 0, 0);
-                own(b * UB + PD * UB, nbl[s], wl[s]);
-            }
+            consume(nbl[s], wl[s], false, 0);
+            // keep the reloads after the last use of the registers they overwrite: hoisted
+            // above it they need fresh registers and loop-end copies that wait for every load
+            __builtin_amdgcn_sched_barrier(0);
+            own((b0 + s + PD) * UB, nbl[s], wl[s]);
         });
     }
+    StaticFor<0, PD>::run([&](auto S) {
+        constexpr int s = decltype(S)::value;
+        if (nfull + s < nbatch) consume(nbl[s], wl[s], true, (nfull + s) * UB);
+    });
 #pragma unroll
     for (int o = 0; o < NOPS; o++)
 #pragma unroll
