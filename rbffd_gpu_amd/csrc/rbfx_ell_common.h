@@ -150,8 +150,9 @@ __device__ __forceinline__ void load_w(const WT* __restrict__ p, WT (&w)[NOPS]) 
 
 // Weights of one (slot,row). FP32 weights carry the streaming (non-temporal) hint: they are
 // read once per stage and should not displace the gathered field records from the caches
-// (measured: 3 % faster stages); for FP64 weights the hint measured 13 % slower (and again
-// 14 % slower with the LDS-staged stage kernel), so they load normally.
+// (measured: 3 % faster stages). FP64 weights of multi-operator sets load normally (the hint
+// measured 13-14 % slower stages); a single FP64 operator (the SpMV) takes the hint, which
+// measured 9 % faster (124 -> 114 us at N = 1M, n = 50).
 template <typename WT, int NOPS>
 __device__ __forceinline__ void load_w_nt(const WT* __restrict__ p, WT (&w)[NOPS]) {
     typedef float f4 __attribute__((ext_vector_type(4)));
@@ -161,6 +162,8 @@ __device__ __forceinline__ void load_w_nt(const WT* __restrict__ p, WT (&w)[NOPS
     } else if constexpr (std::is_same<WT, float>::value) {
 #pragma unroll
         for (int o = 0; o < NOPS; o++) w[o] = __builtin_nontemporal_load(p + o);
+    } else if constexpr (NOPS == 1) {
+        w[0] = __builtin_nontemporal_load(p);
     } else {
         load_w<WT, NOPS>(p, w);
     }
