@@ -22,3 +22,11 @@ int rbfx_build_local_tables(const int32_t* idx_layout, int n_rows, int n_pad, in
 // rows[0..n_rows) only, written at out_idx[row * k ..]; k <= n_nodes, rows in range.
 extern "C" __attribute__((visibility("hidden"))) int rbfx_knn_kdtree_rows(int32_t n_nodes, const double* xyz, int32_t k, const int32_t* rows,
                                     int32_t n_rows, int32_t* out_idx);
+
+// Uniform-grid binning of the nodes for the GPU search of rbfx_knn_gpu: bounding box corner
+// lo, cell size h, cell counts g, cell start table (size g0*g1*g2 + 1) and the nodes sorted
+// by cell (coordinates pts, ids pid). RBFX_E_INVALID for non-finite coordinates.
+__attribute__((visibility("hidden"))) int rbfx_knn_grid_bin(int32_t n, const double* xyz, int32_t k, double lo[3],
+                                                            double* h_out, int32_t g_out[3],
+                                                            std::vector<int32_t>& cstart, std::vector<double>& pts,
+                                                            std::vector<int32_t>& pid);

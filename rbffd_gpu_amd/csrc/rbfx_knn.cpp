@@ -267,6 +267,122 @@ inline double unit_sym(uint64_t r) { return (double)(r >> 11) * (2.0 / 900719925
 
 }  // namespace
 
+// Uniform grid of cubic cells for the GPU k-nearest-neighbour search (rbfx_knn_gpu): bounding
+// box lo..; cell size h halved from the largest extent until a non-empty cell holds about
+// (k+1)/6 nodes (at least 2) or the dense table would pass 2^25 cells; dims g; nodes sorted
+// by cell (cell index (z * gy + y) * gx + x, cell c = sorted nodes cstart[c] .. cstart[c+1],
+// coordinates pts, node ids pid). The order of the nodes inside a cell is not fixed (parallel
+// scatter); the search result does not depend on it. Returns RBFX_E_INVALID for a
+// non-finite coordinate.
+int rbfx_knn_grid_bin(int32_t n, const double* xyz, int32_t k, double lo[3], double* h_out, int32_t g_out[3],
+                      std::vector<int32_t>& cstart, std::vector<double>& pts, std::vector<int32_t>& pid) {
+    double l0 = DBL_MAX, l1 = DBL_MAX, l2 = DBL_MAX, u0 = -DBL_MAX, u1 = -DBL_MAX, u2 = -DBL_MAX;
+    int bad = 0;
+#ifdef _OPENMP
+#pragma omp parallel for reduction(min : l0, l1, l2) reduction(max : u0, u1, u2) reduction(+ : bad)
+#endif
+    for (int32_t i = 0; i < n; i++) {
+        const double* p = xyz + 3 * (size_t)i;
+        bad += !(std::isfinite(p[0]) && std::isfinite(p[1]) && std::isfinite(p[2]));
+        l0 = std::min(l0, p[0]); l1 = std::min(l1, p[1]); l2 = std::min(l2, p[2]);
+        u0 = std::max(u0, p[0]); u1 = std::max(u1, p[1]); u2 = std::max(u2, p[2]);
+    }
+    if (bad) return RBFX_E_INVALID;
+    lo[0] = l0; lo[1] = l1; lo[2] = l2;
+    const double hi[3] = {u0, u1, u2};
+    auto cell_of = [&](const double* p, const int* gg, double inv) {
+        size_t c = 0;
+        for (int d = 2; d >= 0; d--) {
+            int cd = (int)((p[d] - lo[d]) * inv);
+            cd = cd < 0 ? 0 : (cd >= gg[d] ? gg[d] - 1 : cd);
+            c = c * gg[d] + cd;
+        }
+        return c;
+    };
+    double ext = std::max(hi[0] - lo[0], std::max(hi[1] - lo[1], hi[2] - lo[2]));
+    if (!(ext > 0.0)) ext = 1.0;
+    const double target = std::max(2.0, (k + 1) / 6.0);
+    int g[3] = {1, 1, 1};
+    double h = ext;
+    {
+        std::vector<uint8_t> seen;
+        for (int G = 4; G <= 4096; G *= 2) {
+            const double hh = ext * (1.0 + 1e-12) / G;
+            int gg[3];
+            for (int d = 0; d < 3; d++) gg[d] = std::max(1, std::min(G, (int)std::ceil((hi[d] - lo[d]) / hh)));
+            const size_t total = (size_t)gg[0] * gg[1] * gg[2];
+            if (total > ((size_t)1 << 25)) break;
+            seen.assign(total, 0);
+            const double inv = 1.0 / hh;
+#ifdef _OPENMP
+#pragma omp parallel for
+#endif
+            for (int32_t i = 0; i < n; i++)
+                __atomic_store_n(&seen[cell_of(xyz + 3 * (size_t)i, gg, inv)], (uint8_t)1, __ATOMIC_RELAXED);
+            long long occupied = 0;
+#ifdef _OPENMP
+#pragma omp parallel for reduction(+ : occupied)
+#endif
+            for (long long c = 0; c < (long long)total; c++) occupied += seen[(size_t)c];
+            h = hh;
+            std::memcpy(g, gg, sizeof(g));
+            if ((double)n / (double)occupied <= target) break;
+        }
+    }
+    const double inv_h = 1.0 / h;
+    const size_t n_cells = (size_t)g[0] * g[1] * g[2];
+    std::vector<int32_t> cell((size_t)n);
+    cstart.assign(n_cells + 1, 0);
+#ifdef _OPENMP
+#pragma omp parallel for
+#endif
+    for (int32_t i = 0; i < n; i++) {
+        cell[(size_t)i] = (int32_t)cell_of(xyz + 3 * (size_t)i, g, inv_h);
+        __atomic_fetch_add(&cstart[(size_t)cell[(size_t)i] + 1], 1, __ATOMIC_RELAXED);
+    }
+    // inclusive scan of cstart[1..n_cells]: chunk sums, then chunk offsets
+    int nt = 1;
+#ifdef _OPENMP
+    nt = omp_get_max_threads();
+#endif
+    std::vector<long long> part((size_t)nt + 1, 0);
+#ifdef _OPENMP
+#pragma omp parallel num_threads(nt)
+#endif
+    {
+        int t = 0, nth = 1;
+#ifdef _OPENMP
+        t = omp_get_thread_num();
+        nth = omp_get_num_threads();  // <= nt
+#endif
+        const size_t chunk = (n_cells + nth - 1) / nth;
+        const size_t a = std::min(n_cells + 1, 1 + (size_t)t * chunk), b = std::min(n_cells + 1, a + chunk);
+        long long run = 0;
+        for (size_t c = a; c < b; c++) run += cstart[c], cstart[c] = (int32_t)run;
+        part[(size_t)t + 1] = run;
+#ifdef _OPENMP
+#pragma omp barrier
+#pragma omp single
+#endif
+        for (int q = 0; q < nth; q++) part[(size_t)q + 1] += part[(size_t)q];
+        for (size_t c = a; c < b; c++) cstart[c] += (int32_t)part[(size_t)t];
+    }
+    std::vector<int32_t> fill(cstart.begin(), cstart.end() - 1);
+    pts.resize(3 * (size_t)n);
+    pid.resize((size_t)n);
+#ifdef _OPENMP
+#pragma omp parallel for
+#endif
+    for (int32_t i = 0; i < n; i++) {
+        const int32_t s = __atomic_fetch_add(&fill[(size_t)cell[(size_t)i]], 1, __ATOMIC_RELAXED);
+        std::memcpy(&pts[3 * (size_t)s], &xyz[3 * (size_t)i], 3 * sizeof(double));
+        pid[(size_t)s] = i;
+    }
+    *h_out = h;
+    std::memcpy(g_out, g, sizeof(g));
+    return RBFX_OK;
+}
+
 int rbfx_build_local_tables(const int32_t* idx_layout, int n_rows, int n_pad, int width, int tile_layout, int lt,
                             int cap, std::vector<int32_t>& pos, std::vector<int32_t>& off, std::vector<int32_t>& nodes,
                             std::vector<uint16_t>& loc_idx) {

@@ -132,72 +132,24 @@ int rbfx_knn_gpu(rbfx_ctx* ctx, int32_t n_nodes, const double* xyz, int32_t k, i
     if (n_nodes <= 0 || !xyz || !out_idx || k <= 0 || k > n_nodes)
         return rbfx_fail(ctx, RBFX_E_INVALID, "knn_gpu: bad arguments (n_nodes %d, k %d)", n_nodes, k);
     if (n_host_rows) *n_host_rows = 0;
-    double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
-    for (size_t i = 0; i < 3 * (size_t)n_nodes; i++) {
-        if (!std::isfinite(xyz[i])) return rbfx_fail(ctx, RBFX_E_INVALID, "knn_gpu: non-finite coordinate");
-        lo[i % 3] = std::min(lo[i % 3], xyz[i]);
-        hi[i % 3] = std::max(hi[i % 3], xyz[i]);
-    }
     // the GPU list holds k+1 entries per query in LDS (64 or 32 queries per block)
     const int B = (size_t)(k + 1) * 64 * 12 <= 65536 ? 64 : 32;
     if (k + 1 > n_nodes || (size_t)(k + 1) * B * 12 > 65536) {
+        for (size_t i = 0; i < 3 * (size_t)n_nodes; i++)
+            if (!std::isfinite(xyz[i])) return rbfx_fail(ctx, RBFX_E_INVALID, "knn_gpu: non-finite coordinate");
         std::vector<int32_t> rows((size_t)n_nodes);
         for (int32_t i = 0; i < n_nodes; i++) rows[(size_t)i] = i;
         if (n_host_rows) *n_host_rows = n_nodes;
         return rbfx_knn_kdtree_rows(n_nodes, xyz, k, rows.data(), n_nodes, out_idx);
     }
-    // cubic cells: halve the cell size until a non-empty cell holds about (k+1)/6 points
-    // (at least 2), or the dense cell table would pass 2^25 cells
-    double ext = std::max(hi[0] - lo[0], std::max(hi[1] - lo[1], hi[2] - lo[2]));
-    if (!(ext > 0.0)) ext = 1.0;
-    const double target = std::max(2.0, (k + 1) / 6.0);
-    int g[3] = {1, 1, 1};
-    double h = ext;
-    std::vector<uint8_t> seen;
-    for (int G = 4; G <= 4096; G *= 2) {
-        const double hh = ext * (1.0 + 1e-12) / G;
-        int gg[3];
-        for (int d = 0; d < 3; d++) gg[d] = std::max(1, std::min(G, (int)std::ceil((hi[d] - lo[d]) / hh)));
-        const size_t total = (size_t)gg[0] * gg[1] * gg[2];
-        if (total > ((size_t)1 << 25)) break;
-        seen.assign(total, 0);
-        size_t occupied = 0;
-        for (int32_t i = 0; i < n_nodes; i++) {
-            size_t c = 0;
-            for (int d = 2; d >= 0; d--) {
-                int cd = (int)((xyz[3 * (size_t)i + d] - lo[d]) * (1.0 / hh));
-                cd = cd < 0 ? 0 : (cd >= gg[d] ? gg[d] - 1 : cd);
-                c = c * gg[d] + cd;
-            }
-            occupied += !seen[c];
-            seen[c] = 1;
-        }
-        h = hh;
-        std::memcpy(g, gg, sizeof(g));
-        if ((double)n_nodes / (double)occupied <= target) break;
-    }
-    std::vector<uint8_t>().swap(seen);
+    double lo[3], h = 1.0;
+    int32_t g[3];
+    std::vector<int32_t> cstart, pid;
+    std::vector<double> pts;
+    if (rbfx_knn_grid_bin(n_nodes, xyz, k, lo, &h, g, cstart, pts, pid) != RBFX_OK)
+        return rbfx_fail(ctx, RBFX_E_INVALID, "knn_gpu: non-finite coordinate");
     const double inv_h = 1.0 / h;
     const size_t n_cells = (size_t)g[0] * g[1] * g[2];
-    // counting sort of the points by cell (cell index = (z * gy + y) * gx + x)
-    std::vector<int32_t> cstart(n_cells + 1, 0), cell((size_t)n_nodes);
-    for (int32_t i = 0; i < n_nodes; i++) {
-        int c3[3];
-        for (int d = 0; d < 3; d++) {
-            int cd = (int)((xyz[3 * (size_t)i + d] - lo[d]) * inv_h);
-            c3[d] = cd < 0 ? 0 : (cd >= g[d] ? g[d] - 1 : cd);
-        }
-        cell[(size_t)i] = (int32_t)(((size_t)c3[2] * g[1] + c3[1]) * g[0] + c3[0]);
-        cstart[(size_t)cell[(size_t)i] + 1]++;
-    }
-    for (size_t c = 0; c < n_cells; c++) cstart[c + 1] += cstart[c];
-    std::vector<double> pts(3 * (size_t)n_nodes);
-    std::vector<int32_t> pid((size_t)n_nodes), fill(cstart.begin(), cstart.end() - 1);
-    for (int32_t i = 0; i < n_nodes; i++) {
-        const int32_t s = fill[(size_t)cell[(size_t)i]]++;
-        std::memcpy(&pts[3 * (size_t)s], &xyz[3 * (size_t)i], 3 * sizeof(double));
-        pid[(size_t)s] = i;
-    }
     RBFX_HIP_CHECK(ctx, hipSetDevice(ctx->device));
     hipStream_t st = ctx->stream;
     DevBufs db;
