@@ -153,14 +153,16 @@ def stream_copy_gbs(ctx, R, nbytes=1 << 30, reps=10):
 C3_EPS = {17: (0.026, 0.08), 31: (0.035, 0.1), 50: (0.044, 0.14), 75: (0.051, 0.15), 101: (0.058, 0.16)}
 
 
-def bench_weights(ctx, R, N, cpu, nthreads, degrees=(2, 3), modes=("f64_lu", "f32_lu_f64_ir")):
+def bench_weights(ctx, R, N, cpu, nthreads, degrees=(2, 3), modes=("f64_lu", "f32_lu_f64_ir"), eps_scale=1.0):
     """Batched weights with uneven stencils: N jittered sphere nodes, each stencil the n nearest
     with n drawn uniformly from {17, 31, 50, 75, 101} (seed 1), eps = c1 sqrt(N) - c2 per size
     (the reference's table, n = 75 interpolated), X/Y/Z/LAPL (dim 3), degree 2 and 3; FP64 LU and
     FP32 LU + FP64 refinement, each one timed call over all stencils without diagnostics (the
     production path), then an untimed call with diagnostics for the backward errors. With cpu, the C++ restatement
     times a bounded sample of the same stencils on the host threads and checks the GPU weights
-    of that sample (backward error, forward difference against the per-stencil condition)."""
+    of that sample (backward error, forward difference against the per-stencil condition).
+    eps_scale multiplies every eps (1: the reference's table; larger: less flat, better
+    conditioned systems)."""
     t0 = time.time()
     xyz = R.nodes_fibonacci(N, 1e-3, 0)
     knn = R.knn_kdtree(xyz, 101)
@@ -169,11 +171,11 @@ def bench_weights(ctx, R, N, cpu, nthreads, degrees=(2, 3), modes=("f64_lu", "f3
     mask_cols = np.arange(101)[None, :] < sizes[:, None]
     idx = knn[mask_cols].astype(np.int32)
     eps = np.array([C3_EPS[k][0] * np.sqrt(N) - C3_EPS[k][1] for k in (17, 31, 50, 75, 101)])[
-        np.searchsorted([17, 31, 50, 75, 101], sizes)]
+        np.searchsorted([17, 31, 50, 75, 101], sizes)] * eps_scale
     log(f"weights workload: {N} stencils, {int(off[-1])} nonzeros, setup {time.time() - t0:.1f}s")
     d_xyz, d_off, d_idx = ctx.to_device(xyz), ctx.to_device(off), ctx.to_device(idx)
     mask = R.X | R.Y | R.Z | R.LAPL
-    res = {"N": N, "sizes": [17, 31, 50, 75, 101], "operators": "X,Y,Z,LAPL (dim 3)"}
+    res = {"N": N, "sizes": [17, 31, 50, 75, 101], "operators": "X,Y,Z,LAPL (dim 3)", "eps_scale": eps_scale}
     outs = {}
     all_modes = {"f64_lu": R.SOLVE_F64_LU, "f32_lu_f64_ir": R.SOLVE_F32_LU_F64_IR}
     for deg in degrees:
@@ -251,6 +253,8 @@ def main():
     ap.add_argument("--kernel-only", action="store_true", help="set up, then launch only stage kernels (profiling)")
     ap.add_argument("--no-weights", action="store_true", help="skip the batched uneven-stencil weights section")
     ap.add_argument("--no-revolution", action="store_true", help="skip the untimed one-revolution accuracy run")
+    ap.add_argument("--weights-eps-scale", type=float, default=1.0,
+                    help="with --weights-only: multiply the weights workload's eps (better conditioned when > 1)")
     ap.add_argument("--weights-only", action="store_true",
                     help="run only the batched uneven-stencil weights section (profiling)")
     ap.add_argument("--weights-f64-deg2", action="store_true",
@@ -272,7 +276,7 @@ def main():
     if args.weights_only:
         ctx = R.Context(0)
         sel = dict(degrees=(2,), modes=("f64_lu",)) if args.weights_f64_deg2 else {}
-        res = bench_weights(ctx, R, N, False, 1, **sel)
+        res = bench_weights(ctx, R, N, False, 1, eps_scale=args.weights_eps_scale, **sel)
         log("weights-only done", json.dumps(res))
         return
     t0 = time.time()
