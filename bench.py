@@ -338,10 +338,11 @@ def main():
             e.set_row_order(order)
         log(f"Morton row order: {time.time() - t0:.2f}s")
         t0 = time.time()
-        # node cap per tile: 28 KB (FP32) / 36 KB (FP64) of staged records per block, the
-        # fastest caps of a sweep over 16-36 KB and 32- or 64-row tiles
-        for name, e, fb, cap in (("fp64", ell64, 8, 36 * 1024), ("fp32", ell32, 4, 28 * 1024)):
-            tl, k_max, n_tiles = e.build_local(64, cap // (F * fb))
+        # tiles and node caps: FP32 128-row tiles with 48 KB of staged records per block (three
+        # 512-thread blocks per CU), FP64 64-row tiles with 36 KB; the fastest of sweeps over
+        # 32-, 64- and 128-row tiles and 16-56 KB caps
+        for name, e, fb, rows, cap in (("fp64", ell64, 8, 64, 36 * 1024), ("fp32", ell32, 4, 128, 48 * 1024)):
+            tl, k_max, n_tiles = e.build_local(rows, cap // (F * fb))
             log(f"LDS staging tables {name}: {n_tiles} tiles of <= {tl} rows, at most {k_max} distinct nodes per tile")
         log(f"LDS staging tables: {time.time() - t0:.2f}s")
     t0 = time.time()

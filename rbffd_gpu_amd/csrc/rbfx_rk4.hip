@@ -90,11 +90,12 @@ __global__ void __launch_bounds__(256) rk4_stage_kernel(StageArgs a) {
     stage_update<UT, AT, NOPS, FPT>(a, row, fofs, d);
 }
 
-// LDS-staged variant: block = one tile of a.lt positions (a.lt * G threads); the tile's
-// distinct neighbour records are copied into LDS first, then every row gathers from there.
+// LDS-staged variant: block = one tile of a.lt positions (a.lt * G threads, up to 512: 128-row
+// tiles with 4 lanes per row); the tile's distinct neighbour records are copied into LDS
+// first, then every row gathers from there.
 // MX: FP32 multiply-adds on the matrix cores (16 fields, 4 lanes per row).
 template <typename WT, typename UT, typename AT, int NOPS, int FPT, int UB, int SG, int PD, bool MX = false>
-__global__ void __launch_bounds__(256) rk4_stage_lds_kernel(StageArgs a) {
+__global__ void __launch_bounds__(512) rk4_stage_lds_kernel(StageArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     UT* lds = reinterpret_cast<UT*>(smem);
     const int tile = xcd_tile(blockIdx.x, a.nb);

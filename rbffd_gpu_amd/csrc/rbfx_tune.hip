@@ -67,7 +67,7 @@ int rbfx_tune_set(rbfx_ell* e, int32_t kind, int32_t n_fields, int32_t field_typ
         const size_t rec = (size_t)n_fields * (field_type == RBFX_F64 ? 8 : 4);
         bool ok = kind == 1 && e->lt > 0 && e->n_ops >= 3 && fpt > 0 && n_fields % fpt == 0 &&
                   ((fpt == 4 && G == 4) || (fpt == 8 && G == 2)) && (batch == 4 || batch == 8) &&
-                  e->lt * G <= 256 && rec % 16 == 0 && (size_t)e->k_max * rec <= (size_t)e->lds_limit;
+                  e->lt * G <= 512 && rec % 16 == 0 && (size_t)e->k_max * rec <= (size_t)e->lds_limit;
         // matrix-core multiply-adds: FP32 weights, fields and accumulation, 16 fields on 4 lanes
         if (share == 5)
             ok = ok && fpt == 4 && G == 4 && e->storage != RBFX_F64 && field_type == RBFX_F32 && accum == RBFX_F32;

@@ -31,7 +31,7 @@ def candidates(n_fields, kind=KIND_SPMM, n_ops=1, local=None, field_bytes=8, fp3
     if kind == KIND_RK4 and local and local[0] > 0 and n_ops >= 3 and (n_fields * field_bytes) % 16 == 0 \
             and local[1] * n_fields * field_bytes <= LDS_BYTES:
         for fpt, G in ((4, 4), (8, 2)):
-            if n_fields == fpt * G and local[0] * G <= 256:
+            if n_fields == fpt * G and local[0] * G <= 512:
                 for batch in (4, 8):
                     yield fpt, local[0] * G, batch, 4
                     if fp32_accum and field_bytes == 4 and fpt == 4:

@@ -128,13 +128,14 @@ def test_rk4_amplification_factor(ctx):
     ell.close()
 
 
-@pytest.mark.parametrize("n_ops", [3, 4])
-def test_rk4_matrix_core_fp32_bit_identical(ctx, c1, n_ops):
+@pytest.mark.parametrize("n_ops,rows", [(3, 64), (4, 64), (4, 128)])
+def test_rk4_matrix_core_fp32_bit_identical(ctx, c1, n_ops, rows):
     """FP32 weights, fields and accumulation: the LDS-staged stage kernel whose multiply-adds
     run on the matrix cores (4x4x1 FP32 MFMA, one 4x4 block per row) takes the slots in the
     same order with one rounding per multiply-add, so its states are bit-identical to the
-    VALU kernel's; with 3 operators the fourth lane of each row feeds zeros."""
-    print(f"case: rk4 matrix-core fp32 n_ops={n_ops}", flush=True)
+    VALU kernel's; with 3 operators the fourth lane of each row feeds operator 0's weights
+    into the discarded fourth accumulator row. 128-row tiles run 512-thread blocks."""
+    print(f"case: rk4 matrix-core fp32 n_ops={n_ops} tile rows {rows}", flush=True)
     xyz, st, ws = _c1_ops(c1)
     ws = ws[:n_ops]
     F = 16
@@ -142,7 +143,7 @@ def test_rk4_matrix_core_fp32_bit_identical(ctx, c1, n_ops):
     coefs = [xyz[:, 1], -xyz[:, 0], 0.1 * xyz[:, 2], None][:n_ops]
     ell = R.EllOperators(ctx, st, ws, storage=R.F32)
     ell.set_row_order(R.order_morton(xyz))
-    tl, k_max, n_tiles = ell.build_local(64, 128)  # variable tile sizes, partly filled blocks
+    tl, k_max, n_tiles = ell.build_local(rows, 2 * rows)  # variable tile sizes, partly filled blocks
     rk = R.RK4(ctx, ell, F, R.F32, R.F32, coefs=coefs)
     from rbffd_gpu_amd import tune as TU
     shapes = [c for c in TU.candidates(F, TU.KIND_RK4, n_ops, (tl, k_max, n_tiles), 4, True) if c[3] in (4, 5)]
