@@ -299,12 +299,20 @@ def main():
     t0 = time.time()
     st_gpu, knn_host_rows = R.knn_gpu(ctx, xyz, n)
     t_knn_gpu = time.time() - t0
+    # and straight into device memory (no copy of the lists back to the host)
+    ctx.sync()
+    t0 = time.time()
+    d_st_gpu, _ = R.knn_gpu_device(ctx, xyz, n)
+    ctx.sync()
+    t_knn_dev = time.time() - t0
     knn = {"n_nodes": N, "k": n, "kdtree_host_s": round(t_kdtree, 3), "gpu_s": round(t_knn_gpu, 3),
-           "host_rows": knn_host_rows, "identical_to_kdtree": bool(np.array_equal(st_gpu, st)),
+           "gpu_to_device_s": round(t_knn_dev, 3), "host_rows": knn_host_rows,
+           "identical_to_kdtree": bool(np.array_equal(st_gpu, st) and np.array_equal(d_st_gpu.download(), st)),
            "host_threads": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count()))}
+    d_st_gpu.free()
     del st_gpu
-    log(f"GPU kNN stencils: {t_knn_gpu:.3f}s ({knn_host_rows} rows from the host kd-tree), "
-        f"identical to the kd-tree: {knn['identical_to_kdtree']}")
+    log(f"GPU kNN stencils: {t_knn_gpu:.3f}s to host memory, {t_knn_dev:.3f}s to device memory "
+        f"({knn_host_rows} rows from the host kd-tree), identical to the kd-tree: {knn['identical_to_kdtree']}")
 
     # ---- weights on the GPU ----------------------------------------------------------------
     mask = R.XSFC | R.YSFC | R.ZSFC | R.HV  # columns in ascending bit order: HV, XSFC, YSFC, ZSFC

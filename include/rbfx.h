@@ -112,6 +112,12 @@ int rbfx_knn_kdtree(int32_t n_nodes, const double* xyz, int32_t k, int32_t* out_
  * those) are recomputed on the host by the kd-tree; n_host_rows (may be NULL) receives how
  * many. The result is bit-identical to rbfx_knn_kdtree. k <= n_nodes. */
 int rbfx_knn_gpu(rbfx_ctx* ctx, int32_t n_nodes, const double* xyz, int32_t k, int32_t* out_idx, int32_t* n_host_rows);
+/* rbfx_knn_gpu with the lists written to device memory instead (d_out_idx: n_nodes * k int32
+ * on the context's device; xyz stays a host array), so that rbfx_weights and rbfx_ell_create
+ * can take them without a round trip through host memory. Tie rows are recomputed on the
+ * host and uploaded. No counterpart in the reference (its stencils live on the host). */
+int rbfx_knn_gpu_device(rbfx_ctx* ctx, int32_t n_nodes, const double* xyz, int32_t k, int32_t* d_out_idx,
+                        int32_t* n_host_rows);
 /* Seeded node set on the unit sphere: golden-angle (Fibonacci) spiral plus an independent
  * uniform +-jitter per coordinate (SplitMix64 counter stream), not re-projected.
  * Stands in for GridReader::readNodeList's input files (src/grids/grid_reader.cpp:36-100). */

@@ -18,6 +18,7 @@ from .lib import (  # noqa: F401
     SOLVE_F64_LU, SOLVE_F32_LU_F64_IR, POLY_SPHERE,
     knn_kdtree,
     knn_gpu,
+    knn_gpu_device,
     nodes_fibonacci,
     nodes_icosahedral,
     order_morton,

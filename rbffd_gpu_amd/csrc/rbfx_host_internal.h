@@ -19,9 +19,10 @@ int rbfx_build_local_tables(const int32_t* idx_layout, int n_rows, int n_pad, in
                             std::vector<uint16_t>& loc_idx);
 
 // kd-tree k-nearest-neighbour rows (same answers as rbfx_knn_kdtree) for the query rows
-// rows[0..n_rows) only, written at out_idx[row * k ..]; k <= n_nodes, rows in range.
+// rows[0..n_rows) only, written at out_idx[row * k ..] (compact: at out_idx[r * k ..] for the
+// r-th listed row); k <= n_nodes, rows in range.
 extern "C" __attribute__((visibility("hidden"))) int rbfx_knn_kdtree_rows(int32_t n_nodes, const double* xyz, int32_t k, const int32_t* rows,
-                                    int32_t n_rows, int32_t* out_idx);
+                                    int32_t n_rows, int32_t* out_idx, int32_t compact);
 
 // Uniform-grid binning of the nodes for the GPU search of rbfx_knn_gpu: bounding box corner
 // lo, cell size h, cell counts g, cell start table (size g0*g1*g2 + 1) and the nodes sorted

@@ -463,10 +463,11 @@ int rbfx_knn_kdtree(int32_t n_nodes, const double* xyz, int32_t k, int32_t* out_
     return RBFX_OK;
 }
 
-// kd-tree answers for the listed query rows only (out_idx keeps the n_nodes x k layout; other
-// rows are left untouched). Used for the rows that the GPU search hands back.
+// kd-tree answers for the listed query rows only. compact = 0: out_idx keeps the n_nodes x k
+// layout (other rows are left untouched); compact = 1: the r-th listed row goes to
+// out_idx[r * k ..]. Used for the rows that the GPU search hands back.
 int rbfx_knn_kdtree_rows(int32_t n_nodes, const double* xyz, int32_t k, const int32_t* rows, int32_t n_rows,
-                         int32_t* out_idx) {
+                         int32_t* out_idx, int32_t compact) {
     if (n_rows <= 0) return RBFX_OK;
     KdTree3 tree(xyz, n_nodes);
 #ifdef _OPENMP
@@ -479,7 +480,7 @@ int rbfx_knn_kdtree_rows(int32_t n_nodes, const double* xyz, int32_t k, const in
 #endif
         for (int32_t r = 0; r < n_rows; r++) {
             const int32_t i = rows[r];
-            tree.knn(xyz + 3 * (size_t)i, k, pq, out_idx + (size_t)i * k);
+            tree.knn(xyz + 3 * (size_t)i, k, pq, out_idx + (size_t)(compact ? r : i) * k);
         }
     }
     return RBFX_OK;

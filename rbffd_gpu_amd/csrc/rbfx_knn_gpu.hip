@@ -118,6 +118,87 @@ struct DevBufs {
     }
 };
 
+// The search with its output either in host memory (out_host) or in device memory (d_out).
+int knn_gpu_impl(rbfx_ctx* ctx, int32_t n_nodes, const double* xyz, int32_t k, int32_t* out_host, int32_t* d_out,
+                 int32_t* n_host_rows) {
+    if (n_host_rows) *n_host_rows = 0;
+    RBFX_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    // host kd-tree answers for `rows`, written to the output wherever it lives
+    auto host_rows = [&](const std::vector<int32_t>& rows) -> int {
+        if (n_host_rows) *n_host_rows = (int32_t)rows.size();
+        if (rows.empty()) return RBFX_OK;
+        if (out_host) return rbfx_knn_kdtree_rows(n_nodes, xyz, k, rows.data(), (int32_t)rows.size(), out_host, 0);
+        std::vector<int32_t> tmp(rows.size() * (size_t)k);
+        int rc = rbfx_knn_kdtree_rows(n_nodes, xyz, k, rows.data(), (int32_t)rows.size(), tmp.data(), 1);
+        if (rc != RBFX_OK) return rc;
+        for (size_t r = 0; r < rows.size(); r++)
+            RBFX_HIP_CHECK(ctx, hipMemcpyAsync(d_out + (size_t)rows[r] * k, tmp.data() + r * k, sizeof(int32_t) * k,
+                                               hipMemcpyHostToDevice, st));
+        RBFX_HIP_CHECK(ctx, hipStreamSynchronize(st));
+        return RBFX_OK;
+    };
+    // the GPU list holds k+1 entries per query in LDS (64 or 32 queries per block)
+    const int B = (size_t)(k + 1) * 64 * 12 <= 65536 ? 64 : 32;
+    if (k + 1 > n_nodes || (size_t)(k + 1) * B * 12 > 65536) {
+        for (size_t i = 0; i < 3 * (size_t)n_nodes; i++)
+            if (!std::isfinite(xyz[i])) return rbfx_fail(ctx, RBFX_E_INVALID, "knn_gpu: non-finite coordinate");
+        std::vector<int32_t> rows((size_t)n_nodes);
+        for (int32_t i = 0; i < n_nodes; i++) rows[(size_t)i] = i;
+        return host_rows(rows);
+    }
+    double lo[3], h = 1.0;
+    int32_t g[3];
+    std::vector<int32_t> cstart, pid;
+    std::vector<double> pts;
+    if (rbfx_knn_grid_bin(n_nodes, xyz, k, lo, &h, g, cstart, pts, pid) != RBFX_OK)
+        return rbfx_fail(ctx, RBFX_E_INVALID, "knn_gpu: non-finite coordinate");
+    const size_t n_cells = (size_t)g[0] * g[1] * g[2];
+    DevBufs db;
+    const size_t sz[6] = {sizeof(double) * 3 * (size_t)n_nodes, sizeof(int32_t) * (size_t)n_nodes,
+                          sizeof(int32_t) * (n_cells + 1), sizeof(double) * 3 * (size_t)n_nodes,
+                          d_out ? 0 : sizeof(int32_t) * (size_t)n_nodes * k, sizeof(int32_t) * (size_t)n_nodes};
+    for (int b = 0; b < 6; b++) {
+        if (!sz[b]) continue;
+        hipError_t e = hipMalloc(&db.p[b], sz[b]);
+        if (e != hipSuccess) return rbfx_fail(ctx, RBFX_E_NOMEM, "knn_gpu: %s", hipGetErrorString(e));
+    }
+    RBFX_HIP_CHECK(ctx, hipMemcpyAsync(db.p[0], pts.data(), sz[0], hipMemcpyHostToDevice, st));
+    RBFX_HIP_CHECK(ctx, hipMemcpyAsync(db.p[1], pid.data(), sz[1], hipMemcpyHostToDevice, st));
+    RBFX_HIP_CHECK(ctx, hipMemcpyAsync(db.p[2], cstart.data(), sz[2], hipMemcpyHostToDevice, st));
+    RBFX_HIP_CHECK(ctx, hipMemcpyAsync(db.p[3], xyz, sz[3], hipMemcpyHostToDevice, st));
+    KnnArgs a;
+    a.pts = (const double*)db.p[0];
+    a.pid = (const int32_t*)db.p[1];
+    a.cstart = (const int32_t*)db.p[2];
+    a.q = (const double*)db.p[3];
+    a.out = d_out ? d_out : (int32_t*)db.p[4];
+    a.flag = (int32_t*)db.p[5];
+    a.n = n_nodes;
+    a.k = k;
+    a.gx = g[0];
+    a.gy = g[1];
+    a.gz = g[2];
+    a.lo0 = lo[0];
+    a.lo1 = lo[1];
+    a.lo2 = lo[2];
+    a.inv_h = 1.0 / h;
+    a.h = h;
+    const size_t lds = (size_t)(k + 1) * B * 12;
+    knn_grid_kernel<<<(n_nodes + B - 1) / B, B, lds, st>>>(a);
+    RBFX_LAUNCH_CHECK(ctx, "knn_grid_kernel");
+    std::vector<int32_t> flag((size_t)n_nodes);
+    if (out_host)
+        RBFX_HIP_CHECK(ctx, hipMemcpyAsync(out_host, db.p[4], sizeof(int32_t) * (size_t)n_nodes * k,
+                                           hipMemcpyDeviceToHost, st));
+    RBFX_HIP_CHECK(ctx, hipMemcpyAsync(flag.data(), db.p[5], sz[5], hipMemcpyDeviceToHost, st));
+    RBFX_HIP_CHECK(ctx, hipStreamSynchronize(st));
+    std::vector<int32_t> rows;
+    for (int32_t i = 0; i < n_nodes; i++)
+        if (flag[(size_t)i]) rows.push_back(i);
+    return host_rows(rows);
+}
+
 }  // namespace
 
 extern "C" {
@@ -131,68 +212,18 @@ int rbfx_knn_gpu(rbfx_ctx* ctx, int32_t n_nodes, const double* xyz, int32_t k, i
     if (!ctx) return RBFX_E_INVALID;
     if (n_nodes <= 0 || !xyz || !out_idx || k <= 0 || k > n_nodes)
         return rbfx_fail(ctx, RBFX_E_INVALID, "knn_gpu: bad arguments (n_nodes %d, k %d)", n_nodes, k);
-    if (n_host_rows) *n_host_rows = 0;
-    // the GPU list holds k+1 entries per query in LDS (64 or 32 queries per block)
-    const int B = (size_t)(k + 1) * 64 * 12 <= 65536 ? 64 : 32;
-    if (k + 1 > n_nodes || (size_t)(k + 1) * B * 12 > 65536) {
-        for (size_t i = 0; i < 3 * (size_t)n_nodes; i++)
-            if (!std::isfinite(xyz[i])) return rbfx_fail(ctx, RBFX_E_INVALID, "knn_gpu: non-finite coordinate");
-        std::vector<int32_t> rows((size_t)n_nodes);
-        for (int32_t i = 0; i < n_nodes; i++) rows[(size_t)i] = i;
-        if (n_host_rows) *n_host_rows = n_nodes;
-        return rbfx_knn_kdtree_rows(n_nodes, xyz, k, rows.data(), n_nodes, out_idx);
-    }
-    double lo[3], h = 1.0;
-    int32_t g[3];
-    std::vector<int32_t> cstart, pid;
-    std::vector<double> pts;
-    if (rbfx_knn_grid_bin(n_nodes, xyz, k, lo, &h, g, cstart, pts, pid) != RBFX_OK)
-        return rbfx_fail(ctx, RBFX_E_INVALID, "knn_gpu: non-finite coordinate");
-    const double inv_h = 1.0 / h;
-    const size_t n_cells = (size_t)g[0] * g[1] * g[2];
-    RBFX_HIP_CHECK(ctx, hipSetDevice(ctx->device));
-    hipStream_t st = ctx->stream;
-    DevBufs db;
-    const size_t sz[6] = {sizeof(double) * 3 * (size_t)n_nodes, sizeof(int32_t) * (size_t)n_nodes,
-                          sizeof(int32_t) * (n_cells + 1), sizeof(double) * 3 * (size_t)n_nodes,
-                          sizeof(int32_t) * (size_t)n_nodes * k, sizeof(int32_t) * (size_t)n_nodes};
-    for (int b = 0; b < 6; b++) {
-        hipError_t e = hipMalloc(&db.p[b], sz[b]);
-        if (e != hipSuccess) return rbfx_fail(ctx, RBFX_E_NOMEM, "knn_gpu: %s", hipGetErrorString(e));
-    }
-    RBFX_HIP_CHECK(ctx, hipMemcpyAsync(db.p[0], pts.data(), sz[0], hipMemcpyHostToDevice, st));
-    RBFX_HIP_CHECK(ctx, hipMemcpyAsync(db.p[1], pid.data(), sz[1], hipMemcpyHostToDevice, st));
-    RBFX_HIP_CHECK(ctx, hipMemcpyAsync(db.p[2], cstart.data(), sz[2], hipMemcpyHostToDevice, st));
-    RBFX_HIP_CHECK(ctx, hipMemcpyAsync(db.p[3], xyz, sz[3], hipMemcpyHostToDevice, st));
-    KnnArgs a;
-    a.pts = (const double*)db.p[0];
-    a.pid = (const int32_t*)db.p[1];
-    a.cstart = (const int32_t*)db.p[2];
-    a.q = (const double*)db.p[3];
-    a.out = (int32_t*)db.p[4];
-    a.flag = (int32_t*)db.p[5];
-    a.n = n_nodes;
-    a.k = k;
-    a.gx = g[0];
-    a.gy = g[1];
-    a.gz = g[2];
-    a.lo0 = lo[0];
-    a.lo1 = lo[1];
-    a.lo2 = lo[2];
-    a.inv_h = inv_h;
-    a.h = h;
-    const size_t lds = (size_t)(k + 1) * B * 12;
-    knn_grid_kernel<<<(n_nodes + B - 1) / B, B, lds, st>>>(a);
-    RBFX_LAUNCH_CHECK(ctx, "knn_grid_kernel");
-    std::vector<int32_t> flag((size_t)n_nodes);
-    RBFX_HIP_CHECK(ctx, hipMemcpyAsync(out_idx, db.p[4], sz[4], hipMemcpyDeviceToHost, st));
-    RBFX_HIP_CHECK(ctx, hipMemcpyAsync(flag.data(), db.p[5], sz[5], hipMemcpyDeviceToHost, st));
-    RBFX_HIP_CHECK(ctx, hipStreamSynchronize(st));
-    std::vector<int32_t> rows;
-    for (int32_t i = 0; i < n_nodes; i++)
-        if (flag[(size_t)i]) rows.push_back(i);
-    if (n_host_rows) *n_host_rows = (int32_t)rows.size();
-    return rbfx_knn_kdtree_rows(n_nodes, xyz, k, rows.data(), (int32_t)rows.size(), out_idx);
+    return knn_gpu_impl(ctx, n_nodes, xyz, k, out_idx, nullptr, n_host_rows);
+}
+
+// The same lists written to device memory (d_out_idx: n_nodes * k int32 on the context's
+// device), so that the weights and operators can be built from them without a round trip
+// through host memory. Tie rows are recomputed on the host and uploaded row by row.
+int rbfx_knn_gpu_device(rbfx_ctx* ctx, int32_t n_nodes, const double* xyz, int32_t k, int32_t* d_out_idx,
+                        int32_t* n_host_rows) {
+    if (!ctx) return RBFX_E_INVALID;
+    if (n_nodes <= 0 || !xyz || !d_out_idx || k <= 0 || k > n_nodes)
+        return rbfx_fail(ctx, RBFX_E_INVALID, "knn_gpu_device: bad arguments (n_nodes %d, k %d)", n_nodes, k);
+    return knn_gpu_impl(ctx, n_nodes, xyz, k, nullptr, d_out_idx, n_host_rows);
 }
 
 }  // extern "C"

@@ -68,6 +68,7 @@ def load_library(path=None):
         "rbfx_timer_stop": (C.c_int, [vp, C.POINTER(f64)]),
         "rbfx_knn_kdtree": (C.c_int, [i32, vp, i32, vp, i32]),
         "rbfx_knn_gpu": (C.c_int, [vp, i32, vp, i32, vp, vp]),
+        "rbfx_knn_gpu_device": (C.c_int, [vp, i32, vp, i32, vp, vp]),
         "rbfx_nodes_fibonacci": (C.c_int, [i32, f64, u64, vp]),
         "rbfx_nodes_icosahedral": (C.c_int, [i32, vp, C.POINTER(i32)]),
         "rbfx_order_morton": (C.c_int, [i32, vp, vp]),
@@ -138,6 +139,16 @@ def knn_gpu(ctx, xyz, k):
     n_host = C.c_int32()
     ctx.check(RBFX.rbfx_knn_gpu(ctx.h, n, _ptr(xyz), k, _ptr(out), C.byref(n_host)), "knn_gpu")
     return out, n_host.value
+
+
+def knn_gpu_device(ctx, xyz, k):
+    """knn_gpu with the lists left in device memory: returns (DeviceArray int32 (N, k), host rows)."""
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    n = xyz.shape[0]
+    d_out = ctx.alloc((n, k), np.int32)
+    n_host = C.c_int32()
+    ctx.check(RBFX.rbfx_knn_gpu_device(ctx.h, n, _ptr(xyz), k, d_out.ptr, C.byref(n_host)), "knn_gpu_device")
+    return d_out, n_host.value
 
 
 def nodes_fibonacci(n, jitter=1e-3, seed=0):

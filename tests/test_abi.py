@@ -55,6 +55,7 @@ def test_invalid_arguments_are_reported_not_fatal():
     assert lib.rbfx_nodes_icosahedral(11, None, C.byref(n)) == -1
     assert lib.rbfx_sync(None) == -1
     assert lib.rbfx_knn_gpu(None, 10, None, 3, None, None) == -1
+    assert lib.rbfx_knn_gpu_device(None, 10, None, 3, None, None) == -1
     assert lib.rbfx_weights(None, 1, None, None, 1, None, None, 1, 3, 0, 0, 1.0, 0, None, None, None) == -1
 
 

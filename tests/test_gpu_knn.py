@@ -91,3 +91,19 @@ def test_knn_gpu_large_sets_digest(ctx):
         print(f"  {n_host} of {rec['n']} rows from the host kd-tree")
         assert np.array_equal(st[:16], np.array(rec["first_rows"], np.int32))
         assert hashlib.sha256(st.tobytes()).hexdigest() == rec["st_sha256"], key
+
+
+def test_knn_gpu_device_output(ctx):
+    """rbfx_knn_gpu_device leaves the same lists in device memory: tie-free rows from the GPU
+    search, tie rows from the host kd-tree uploaded row by row, and k = N entirely from the
+    host kd-tree."""
+    cases = (("fibonacci 20000", R.nodes_fibonacci(20000, 1e-3, 0), 50),
+             ("icosahedral level 4", _ico(4), 31),
+             ("small 60", R.nodes_fibonacci(60, 1e-3, 1), 60))
+    for name, xyz, k in cases:
+        print(f"case: knn_gpu_device {name} k={k}", flush=True)
+        d_st, n_host = R.knn_gpu_device(ctx, xyz, k)
+        got = d_st.download()
+        d_st.free()
+        print(f"  {n_host} of {xyz.shape[0]} rows from the host kd-tree")
+        assert np.array_equal(got, R.knn_kdtree(xyz, k)), name
