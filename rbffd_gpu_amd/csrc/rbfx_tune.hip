@@ -3,8 +3,11 @@
 // A tuning table maps (kind, rows bucket, width, n_ops, fields, storage, field type,
 // accumulation) to the shape measured fastest by rbfx_tune_spmm / the autotuner in the
 // Python layer; the table lives in memory and can be saved/loaded as text so a run can reuse
-// an earlier sweep. Without an entry the heuristic picks the largest fields-per-thread of
-// 8/4/2/1 that divides F and keeps at most 16 FP64 (32 FP32) accumulators per thread.
+// an earlier sweep. Without an entry: an RK4 stage of an operator set with LDS staging tables
+// (rbfx_ell_build_local) that fit takes the LDS-staged shape (4 lanes per row, 16 fields,
+// batch 4; the matrix-core variant for FP32 weights, fields and accumulation); otherwise the
+// heuristic picks the largest fields-per-thread of 8/4/2/1 that divides F and keeps at most
+// 16 FP64 (32 FP32) accumulators per thread.
 #include <cstdio>
 #include <map>
 #include <mutex>
@@ -41,6 +44,14 @@ int rbfx_ell_pick(rbfx_ell* e, int F, int ft, int acc, int kind, EllTune* t) {
         auto it = g_table.find(make_key(e, F, ft, acc, kind));
         if (it != g_table.end()) {
             *t = it->second;
+            return RBFX_OK;
+        }
+    }
+    if (kind == 1 && e->lt > 0 && e->n_ops >= 3 && F == 16 && e->lt * 4 <= 512) {
+        const size_t rec = (size_t)F * (ft == RBFX_F64 ? 8 : 4);
+        if ((size_t)e->k_max * rec <= (size_t)e->lds_limit) {
+            const bool mx = e->storage != RBFX_F64 && ft == RBFX_F32 && acc == RBFX_F32;
+            *t = EllTune{4, e->lt * 4, 4, mx ? 5 : 4};
             return RBFX_OK;
         }
     }

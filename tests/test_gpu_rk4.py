@@ -167,6 +167,36 @@ def test_rk4_matrix_core_fp32_bit_identical(ctx, c1, n_ops, rows):
     ell.close()
 
 
+def test_rk4_default_shape_uses_lds_tables(ctx, c1):
+    """Without a tuning entry, an operator set with LDS staging tables steps with the
+    LDS-staged shape (matrix cores for FP32 weights, fields and accumulation), and its states
+    equal a direct-gather shape's bit for bit."""
+    print("case: rk4 default shape with LDS tables", flush=True)
+    from rbffd_gpu_amd import tune as TU
+    xyz, st, ws = _c1_ops(c1)
+    F = 16
+    u0 = _fields(xyz, F, seed=21)
+    coefs = [xyz[:, 1], -xyz[:, 0], 0.1 * xyz[:, 2], None]
+    for storage, ft, acc, want in ((R.F32, R.F32, R.F32, 5), (R.F32, R.F32, R.F64, 4), (R.F64, R.F64, R.F64, 4)):
+        R.RBFX.rbfx_tune_clear()
+        ell = R.EllOperators(ctx, st, ws, storage=storage)
+        assert ell.tune_get(TU.KIND_RK4, F, ft, acc)[3] == 0  # no tables: direct gathers
+        ell.set_row_order(R.order_morton(xyz))
+        tl, k_max, _ = ell.build_local(128, 256)
+        assert ell.tune_get(TU.KIND_RK4, F, ft, acc) == (4, tl * 4, 4, want)
+        rk = R.RK4(ctx, ell, F, ft, acc, coefs=coefs)
+        rk.set_state(u0)
+        rk.step(1e-3, 2)
+        staged = rk.get_state()
+        ell.tune_set(TU.KIND_RK4, F, ft, acc, 4, 128, 8, 0)
+        rk.set_state(u0)
+        rk.step(1e-3, 2)
+        assert np.array_equal(staged, rk.get_state()), (storage, ft, acc)
+        R.RBFX.rbfx_tune_clear()
+        rk.close()
+        ell.close()
+
+
 def test_rk4_fp32_storage_deviation(ctx, c1):
     """FP32-stored weights and stage fields with FP64 state: deviation from the FP64 path."""
     xyz, st, ws = _c1_ops(c1)
