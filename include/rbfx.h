@@ -234,7 +234,8 @@ int rbfx_rk4_stage(rbfx_rk4* rk, int32_t stage, double dt);
 
 /* ---- launch-shape tuning (fused kernels; kind 0 = SpMM, 1 = RK4 stage) ----------------- */
 /* Fixes fields-per-thread (1,2,4,8), block size (64..256, multiple of 64) and gather batch
- * (4 or 8 stencil slots per memory round trip) for this operator set's row count, width,
+ * (4 or 8 stencil slots per memory round trip; 16 for SpMV/SpMM with one field per thread)
+ * for this operator set's row count, width,
  * operator count, field count and precisions. rbfx_rk4_step picks the shape up when it
  * (re)captures its graph. No counterpart in the reference (its launch shape is fixed). */
 /* share = 1 (RK4 stage kernel only): the G = n_fields/fpt lanes of a row (G = 2 or 4, fpt > 1,

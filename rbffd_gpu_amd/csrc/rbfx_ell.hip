@@ -209,6 +209,7 @@ int launch_spmm_fpt(rbfx_ell* e, const EllTune& t, int F, const void* u, void* o
     }
     RBFX_SPMM_CASE(1, 4) RBFX_SPMM_CASE(2, 4) RBFX_SPMM_CASE(4, 4) RBFX_SPMM_CASE(8, 4)
     RBFX_SPMM_CASE(1, 8) RBFX_SPMM_CASE(2, 8) RBFX_SPMM_CASE(4, 8) RBFX_SPMM_CASE(8, 8)
+    RBFX_SPMM_CASE(1, 16)  // one field per thread (the SpMV): 16 slots per memory round trip
 #undef RBFX_SPMM_CASE
     return rbfx_fail(e->ctx, RBFX_E_INVALID, "spmm: unsupported shape (fields/thread %d, batch %d)", fpt, t.ub);
 }

@@ -90,7 +90,9 @@ int rbfx_tune_set(rbfx_ell* e, int32_t kind, int32_t n_fields, int32_t field_typ
         g_table[make_key(e, n_fields, field_type, accum, kind)] = EllTune{fpt, e->lt * G, batch, share};
         return RBFX_OK;
     }
-    if (!valid(n_fields, fpt, block) || (batch != 4 && batch != 8) || share < 0 || share > 3)
+    // batch 16: the SpMV/SpMM kernel with one field per thread only
+    const bool batch_ok = batch == 4 || batch == 8 || (batch == 16 && kind == 0 && fpt == 1 && share == 0);
+    if (!valid(n_fields, fpt, block) || !batch_ok || share < 0 || share > 3)
         return rbfx_fail(e->ctx, RBFX_E_INVALID, "tune_set: invalid shape fpt %d block %d batch %d share %d for %d fields",
                          fpt, block, batch, share, n_fields);
     if (share) {

@@ -43,7 +43,7 @@ def candidates(n_fields, kind=KIND_SPMM, n_ops=1, local=None, field_bytes=8, fp3
         for block in (64, 128, 256):
             if G > block:
                 continue
-            for batch in (4, 8):
+            for batch in ((4, 8, 16) if (kind == KIND_SPMM and fpt == 1) else (4, 8)):
                 yield fpt, block, batch, 0
                 if kind == KIND_RK4 and fpt > 1 and G in (2, 4) and batch % G == 0:
                     yield fpt, block, batch, 1
