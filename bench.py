@@ -43,33 +43,6 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def real_sph_harm(l, m, xyz):
-    """Real spherical harmonic Y_lm at the directions of xyz (orthonormal on the sphere)."""
-    from scipy.special import sph_harm_y
-
-    r = np.linalg.norm(xyz, axis=1)
-    theta = np.arccos(np.clip(xyz[:, 2] / r, -1, 1))
-    phi = np.arctan2(xyz[:, 1], xyz[:, 0])
-    if m == 0:
-        return sph_harm_y(l, 0, theta, phi).real
-    y = sph_harm_y(l, abs(m), theta, phi)
-    return np.sqrt(2) * (-1) ** m * (y.real if m > 0 else y.imag)
-
-
-def make_fields(xyz, F, seed=0, lmax=20, terms=6):
-    rng = np.random.default_rng(seed)
-    spec = []
-    u = np.zeros((xyz.shape[0], F))
-    for f in range(F):
-        ls = rng.integers(0, lmax + 1, size=terms)
-        ms = [int(rng.integers(-l, l + 1)) for l in ls]
-        cs = rng.standard_normal(terms)
-        spec.append(list(zip(ls.tolist(), ms, cs.tolist())))
-        for l, m, c in spec[-1]:
-            u[:, f] += c * real_sph_harm(int(l), int(m), xyz)
-    return u, spec
-
-
 def rotate(xyz, axis, angle):
     k = axis / np.linalg.norm(axis)
     c, s = np.cos(angle), np.sin(angle)
@@ -354,7 +327,7 @@ def main():
             log(f"LDS staging tables {name}: {n_tiles} tiles of <= {tl} rows, at most {k_max} distinct nodes per tile")
         log(f"LDS staging tables: {time.time() - t0:.2f}s")
     t0 = time.time()
-    u0, spec = make_fields(xyz, F, seed=0)
+    u0, spec = R.make_fields(xyz, F, seed=0)
     log(f"fields: {time.time() - t0:.1f}s")
     axis = np.array([0.3, -0.2, 1.0])
     omega = 2 * np.pi * axis / np.linalg.norm(axis)  # one revolution per unit time
@@ -441,7 +414,7 @@ def main():
     exact = np.zeros_like(u0)
     for f in range(F):
         for l, m, c in spec[f]:
-            exact[:, f] += c * real_sph_harm(int(l), int(m), xr)
+            exact[:, f] += c * R.sph_harm(int(l), int(m), xr)
     err64 = float(np.abs(u64 - exact).max() / np.abs(exact).max())
     err32 = float(np.abs(u32 - exact).max() / np.abs(exact).max())
     dev32a = float(np.abs(u32a - u64).max() / np.abs(u64).max())

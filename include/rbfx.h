@@ -131,6 +131,13 @@ int rbfx_nodes_icosahedral(int32_t level, double* xyz, int32_t* out_n);
  * rbfx_ell_set_row_order. No counterpart in the reference. */
 int rbfx_order_morton(int32_t n_nodes, const double* xyz, int32_t* out_order);
 
+/* Real spherical harmonic Y_lm at the directions of n points (host arrays; out[i] for point i):
+ * orthonormal on the unit sphere, Condon-Shortley phase included, cos(m phi) for m > 0 and
+ * sin(|m| phi) for m < 0 - the normalisation and signs of the closed forms in
+ * src/utils/spherical_harmonics.h:39-170 (Y_3^2, Y_10^5, Y_20^20), for any 0 <= |m| <= l.
+ * Field generator of the multi-field workloads (seeded combinations of degree <= 20). */
+int rbfx_sph_harm(int32_t l, int32_t m, int32_t n, const double* xyz, double* out);
+
 /* ---- batched weight computation ------------------------------------------------------ */
 /* RBFFD::computeAllWeightsForAllStencils -> computeAllWeightsForStencil_Direct
  * (src/rbffd/rbffd.cpp:90-103, 225-352) with getStencilLHS/distanceMatrix (129-191, 888-969)

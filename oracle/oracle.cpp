@@ -458,38 +458,77 @@ void orc_spmm(int32_t n_rows, int32_t n, const int32_t* idx, int32_t n_ops, cons
     (void)n_threads;
 }
 
-namespace {
-// f = sum_o c_o (D_o y); the PDE solve() of a diagonal-coefficient operator combination.
-void rhs_eval(int32_t n_rows, int32_t n, const int32_t* idx, int32_t n_ops, const double* w, const double* const* coef,
-              int32_t F, const double* y, double* f, std::vector<double>& d, int nt) {
-    d.resize((size_t)n_ops * n_rows * F);
-    orc_spmm(n_rows, n, idx, n_ops, w, F, y, d.data(), nt);
-    for (int i = 0; i < n_rows; i++)
-        for (int q = 0; q < F; q++) {
-            double acc = 0.0;
+// Right-hand side of one stage for rows [0, n_rows): f[i][q] = sum_o c_o[i] (D_o y)[i][q], each
+// dot product summed over the stencil in order j = 0..n-1 and the operators combined in order
+// o = 0..n_ops-1 from 0.0 (the rounding of orc_spmm followed by that combination).
+static void rhs_rows(int32_t n_rows, int32_t n, const int32_t* idx, int32_t n_ops, const double* w,
+                     const double* const* coef, int32_t F, const double* y, double* f, int nt) {
+    const size_t WN = (size_t)n_rows * n;
+#ifdef _OPENMP
+#pragma omp parallel num_threads(nt)
+#endif
+    {
+        std::vector<double> dot((size_t)F), acc((size_t)F);
+#ifdef _OPENMP
+#pragma omp for schedule(static)
+#endif
+        for (int i = 0; i < n_rows; i++) {
+            for (int q = 0; q < F; q++) acc[q] = 0.0;
             for (int o = 0; o < n_ops; o++) {
-                double dv = d[(size_t)o * n_rows * F + (size_t)i * F + q];
-                acc += (coef && coef[o]) ? coef[o][i] * dv : dv;
+                const double* wr = w + (size_t)o * WN + (size_t)i * n;
+                const int32_t* ir = idx + (size_t)i * n;
+                for (int q = 0; q < F; q++) dot[q] = 0.0;
+                for (int j = 0; j < n; j++) {
+                    const double wj = wr[j];
+                    const double* yr = y + (size_t)ir[j] * F;
+                    for (int q = 0; q < F; q++) dot[q] += wj * yr[q];
+                }
+                const double c = (coef && coef[o]) ? coef[o][i] : 1.0;
+                for (int q = 0; q < F; q++) acc[q] += (coef && coef[o]) ? c * dot[q] : dot[q];
             }
-            f[(size_t)i * F + q] = acc;
+            for (int q = 0; q < F; q++) f[(size_t)i * F + q] = acc[q];
         }
+    }
 }
-}  // namespace
 
 // TimeDependentPDE::advanceRungeKutta4 (time_dependent_pde.cpp:172-304): unscaled k's,
 // s = u + 0.5*dt*k1, s = u + 0.5*dt*k2, s = u + dt*k3, u += dt/6*(k1 + 2k2 + 2k3 + k4).
+// ws: 5 * n_rows * F doubles of caller-owned workspace (s, k1..k4), reused across steps.
+void orc_rk4_step_ws(int32_t n_rows, int32_t n, const int32_t* idx, int32_t n_ops, const double* w,
+                     const double* const* coef, int32_t F, double dt, double* u, double* ws, int32_t n_threads) {
+    const long long NF = (long long)n_rows * F;
+#ifdef _OPENMP
+    const int nt = n_threads > 0 ? n_threads : omp_get_max_threads();
+#else
+    const int nt = 1;
+#endif
+    double *s = ws, *k1 = ws + NF, *k2 = ws + 2 * NF, *k3 = ws + 3 * NF, *k4 = ws + 4 * NF;
+    rhs_rows(n_rows, n, idx, n_ops, w, coef, F, u, k1, nt);
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(nt) schedule(static)
+#endif
+    for (long long i = 0; i < NF; i++) s[i] = u[i] + 0.5 * dt * (k1[i] + 0.);
+    rhs_rows(n_rows, n, idx, n_ops, w, coef, F, s, k2, nt);
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(nt) schedule(static)
+#endif
+    for (long long i = 0; i < NF; i++) s[i] = u[i] + 0.5 * dt * (k2[i] + 0.);
+    rhs_rows(n_rows, n, idx, n_ops, w, coef, F, s, k3, nt);
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(nt) schedule(static)
+#endif
+    for (long long i = 0; i < NF; i++) s[i] = u[i] + dt * (k3[i] + 0.);
+    rhs_rows(n_rows, n, idx, n_ops, w, coef, F, s, k4, nt);
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(nt) schedule(static)
+#endif
+    for (long long i = 0; i < NF; i++) u[i] = u[i] + (dt / 6.) * (k1[i] + 2. * k2[i] + 2. * k3[i] + k4[i]);
+}
+
 void orc_rk4_step(int32_t n_rows, int32_t n, const int32_t* idx, int32_t n_ops, const double* w,
                   const double* const* coef, int32_t F, double dt, double* u, int32_t n_threads) {
-    size_t NF = (size_t)n_rows * F;
-    std::vector<double> s(NF), k1(NF), k2(NF), k3(NF), k4(NF), d;
-    rhs_eval(n_rows, n, idx, n_ops, w, coef, F, u, k1.data(), d, n_threads);
-    for (size_t i = 0; i < NF; i++) s[i] = u[i] + 0.5 * dt * (k1[i] + 0.);
-    rhs_eval(n_rows, n, idx, n_ops, w, coef, F, s.data(), k2.data(), d, n_threads);
-    for (size_t i = 0; i < NF; i++) s[i] = u[i] + 0.5 * dt * (k2[i] + 0.);
-    rhs_eval(n_rows, n, idx, n_ops, w, coef, F, s.data(), k3.data(), d, n_threads);
-    for (size_t i = 0; i < NF; i++) s[i] = u[i] + dt * (k3[i] + 0.);
-    rhs_eval(n_rows, n, idx, n_ops, w, coef, F, s.data(), k4.data(), d, n_threads);
-    for (size_t i = 0; i < NF; i++) u[i] = u[i] + (dt / 6.) * (k1[i] + 2. * k2[i] + 2. * k3[i] + k4[i]);
+    std::vector<double> ws((size_t)5 * n_rows * F);
+    orc_rk4_step_ws(n_rows, n, idx, n_ops, w, coef, F, dt, u, ws.data(), n_threads);
 }
 
 void orc_cart2sph(double x, double y, double z, double* theta, double* phi, double* r) {

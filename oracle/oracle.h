@@ -60,6 +60,11 @@ void orc_spmm(int32_t n_rows, int32_t n, const int32_t* idx, int32_t n_ops, cons
  * f[i][q] = sum_o c_o[i] (D_o u)[i][q] (c_o NULL -> 1); u updated in place. */
 void orc_rk4_step(int32_t n_rows, int32_t n, const int32_t* idx, int32_t n_ops, const double* w,
                   const double* const* coef, int32_t n_fields, double dt, double* u, int32_t n_threads);
+/* The same step with caller-owned workspace ws of 5 * n_rows * n_fields doubles (s, k1..k4),
+ * so that repeated steps allocate nothing; rows and vector updates run on n_threads. */
+void orc_rk4_step_ws(int32_t n_rows, int32_t n, const int32_t* idx, int32_t n_ops, const double* w,
+                     const double* const* coef, int32_t n_fields, double dt, double* u, double* ws,
+                     int32_t n_threads);
 
 /* Cosine bell (src/pdes/hyperbolic/cosine_bell.{h,cpp}, exact_advection.h). */
 void orc_cart2sph(double x, double y, double z, double* theta, double* phi, double* r);

@@ -16,14 +16,32 @@ LIB = os.path.join(HERE, "liboracle.so")
 REF_KNN = os.path.join(HERE, "_ref", "ref_knn")
 
 _lib = None
+_lib_path = LIB
+
+
+def use_library(path):
+    """Loads the restatement from another build of oracle.cpp (e.g. a -march=native copy made
+    for the host that runs the CPU baseline); must be called before the first use."""
+    global _lib_path
+    if _lib is not None and path != _lib_path:
+        raise RuntimeError("oracle library already loaded from " + _lib_path)
+    _lib_path = path
+
+
+def build_native(out_dir):
+    """Compiles oracle.cpp with -O3 -march=native for this host into out_dir; returns the
+    library path, or None if the compiler is not available."""
+    out = os.path.join(out_dir, "liboracle.so")
+    p = subprocess.run(["make", "-s", "-C", HERE, "native", f"NATIVE_OUT={out}"], capture_output=True, text=True)
+    return out if p.returncode == 0 and os.path.exists(out) else None
 
 
 def lib():
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB):
-            raise OSError(f"{LIB} missing: run make -C oracle")
-        L = C.CDLL(LIB)
+        if not os.path.exists(_lib_path):
+            raise OSError(f"{_lib_path} missing: run make -C oracle")
+        L = C.CDLL(_lib_path)
         d, i32, u32, vp = C.c_double, C.c_int32, C.c_uint32, C.c_void_p
         for name in ("orc_rbf_eval", "orc_rbf_xderiv", "orc_rbf_rderiv_over_r", "orc_rbf_rderiv2"):
             getattr(L, name).restype = d
@@ -50,6 +68,8 @@ def lib():
         L.orc_spmm.argtypes = [i32, i32, vp, i32, vp, i32, vp, vp, i32]
         L.orc_rk4_step.restype = None
         L.orc_rk4_step.argtypes = [i32, i32, vp, i32, vp, vp, i32, d, vp, i32]
+        L.orc_rk4_step_ws.restype = None
+        L.orc_rk4_step_ws.argtypes = [i32, i32, vp, i32, vp, vp, i32, d, vp, vp, i32]
         L.orc_cart2sph.restype = None
         L.orc_cart2sph.argtypes = [d, d, d, C.POINTER(d), C.POINTER(d), C.POINTER(d)]
         L.orc_cosine_bell_coefs.restype = None
@@ -137,21 +157,40 @@ def spmm(idx, ws, u, n_threads=0):
     return out
 
 
+class RK4Stepper:
+    """Repeated RK4 steps (orc_rk4_step_ws) with the weights stacked and the workspace allocated
+    once, outside the steps."""
+
+    def __init__(self, idx, ws, coefs, F, n_threads=0):
+        self.idx = np.ascontiguousarray(idx, np.int32)
+        self.n_rows, self.n = self.idx.shape
+        self.W = np.ascontiguousarray(np.stack([np.asarray(w, np.float64).reshape(self.n_rows, self.n) for w in ws]))
+        self.n_ops = len(ws)
+        self.keep = [None if c is None else np.ascontiguousarray(c, np.float64) for c in (coefs or [None] * len(ws))]
+        self.arr = (C.c_void_p * len(ws))(*[None if c is None else c.ctypes.data for c in self.keep])
+        self.F = F
+        self.work = np.empty(5 * self.n_rows * F)
+        self.n_threads = n_threads
+
+    def step(self, u, dt, n_steps=1):
+        """Advances u (float64, C-contiguous, n_rows x F) in place."""
+        assert u.dtype == np.float64 and u.flags.c_contiguous and u.size == self.n_rows * self.F
+        for _ in range(n_steps):
+            lib().orc_rk4_step_ws(self.n_rows, self.n, _p(self.idx), self.n_ops, _p(self.W), self.arr, self.F, dt,
+                                  _p(u), _p(self.work), self.n_threads)
+
+
 def rk4_step(idx, ws, coefs, u, dt, n_steps=1, n_threads=0, time_budget=None):
     """n_steps RK4 steps; with time_budget (seconds) also stop once the steps taken so far
     have used it, and return (u, steps, seconds in the step loop) instead of u."""
     import time
-    idx = np.ascontiguousarray(idx, np.int32)
-    n_rows, n = idx.shape
-    W = np.ascontiguousarray(np.stack([np.asarray(w, np.float64).reshape(n_rows, n) for w in ws]))
     u = np.array(u, np.float64, order="C")
     F = 1 if u.ndim == 1 else u.shape[1]
-    keep = [None if c is None else np.ascontiguousarray(c, np.float64) for c in (coefs or [None] * len(ws))]
-    arr = (C.c_void_p * len(ws))(*[None if c is None else c.ctypes.data for c in keep])
+    st = RK4Stepper(idx, ws, coefs, F, n_threads)
     t0 = time.perf_counter()
     done = 0
     while done < n_steps:
-        lib().orc_rk4_step(n_rows, n, _p(idx), len(ws), _p(W), arr, F, dt, _p(u), n_threads)
+        st.step(u, dt)
         done += 1
         if time_budget is not None and time.perf_counter() - t0 >= time_budget:
             break

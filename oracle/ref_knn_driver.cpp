@@ -3,9 +3,11 @@
 // (external/kdtree/KDTree.h) compiled from the sources under the reference tree, so that
 // kd-tree stencil lists can be produced by the reference implementation itself.
 //
-// Usage: ref_knn <nodes.bin> <k> <out.bin>
+// Usage: ref_knn <nodes.bin> <k> <out.bin> [grid_name]
 //   nodes.bin : int64 N, then N*3 float64 (x,y,z per node)
 //   out.bin   : int32 N, int32 k, then N*k int32 stencil indices (row 0 = centre)
+//   grid_name : also write the reference's stencil file (Grid::writeStencilsToFile,
+//               grid_interface.cpp:144-170) for that grid name into the working directory
 //
 // Only tests/, bench.py's CPU baseline and the smoke check use this binary, as a checker.
 #include <cstdio>
@@ -16,8 +18,8 @@
 #include "grids/grid_interface.h"
 
 int main(int argc, char** argv) {
-    if (argc != 4) {
-        std::fprintf(stderr, "usage: %s nodes.bin k out.bin\n", argv[0]);
+    if (argc != 4 && argc != 5) {
+        std::fprintf(stderr, "usage: %s nodes.bin k out.bin [grid_name]\n", argv[0]);
         return 2;
     }
     FILE* f = std::fopen(argv[1], "rb");
@@ -38,6 +40,7 @@ int main(int argc, char** argv) {
 
     Grid grid(nodes);
     grid.generateStencils(k, Grid::ST_KDTREE);
+    if (argc == 5) grid.writeStencilsToFile(argv[4]);
 
     std::vector<StencilType>& st = grid.getStencils();
     FILE* o = std::fopen(argv[3], "wb");

@@ -22,5 +22,7 @@ from .lib import (  # noqa: F401
     nodes_fibonacci,
     nodes_icosahedral,
     order_morton,
+    sph_harm,
+    make_fields,
     csr_from_fixed,
 )

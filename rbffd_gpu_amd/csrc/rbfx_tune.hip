@@ -36,6 +36,18 @@ bool valid(int F, int fpt, int block) {
     if (block < 64 || block > 256 || block % 64) return false;
     return (F / fpt) <= block;
 }
+
+// An LDS-staged shape (share 4/5) holds for the operator set it was tuned on: another set with
+// the same key may have no tile tables, other tiles or more nodes per tile than the LDS holds.
+bool lds_shape_fits(const rbfx_ell* e, int F, int ft, int acc, int kind, const EllTune& c) {
+    if (c.fpt <= 0 || F % c.fpt) return false;
+    const int G = F / c.fpt;
+    const size_t rec = (size_t)F * (ft == RBFX_F64 ? 8 : 4);
+    bool ok = kind == 1 && e->lt > 0 && e->n_ops >= 3 && e->lt * G <= 512 && rec % 16 == 0 &&
+              (size_t)e->k_max * rec <= (size_t)e->lds_limit;
+    if (c.share == 5) ok = ok && c.fpt == 4 && G == 4 && e->storage != RBFX_F64 && ft == RBFX_F32 && acc == RBFX_F32;
+    return ok;
+}
 }  // namespace
 
 int rbfx_ell_pick(rbfx_ell* e, int F, int ft, int acc, int kind, EllTune* t) {
@@ -43,8 +55,17 @@ int rbfx_ell_pick(rbfx_ell* e, int F, int ft, int acc, int kind, EllTune* t) {
         std::lock_guard<std::mutex> lk(g_mu);
         auto it = g_table.find(make_key(e, F, ft, acc, kind));
         if (it != g_table.end()) {
-            *t = it->second;
-            return RBFX_OK;
+            EllTune c = it->second;
+            if (c.share != 4 && c.share != 5) {
+                *t = c;
+                return RBFX_OK;
+            }
+            if (lds_shape_fits(e, F, ft, acc, kind, c)) {
+                c.block = e->lt * (F / c.fpt);  // the block is this set's tile
+                *t = c;
+                return RBFX_OK;
+            }
+            // otherwise: the shape of a set without (fitting) tile tables, below
         }
     }
     if (kind == 1 && e->lt > 0 && e->n_ops >= 3 && F == 16 && e->lt * 4 <= 512) {

@@ -84,10 +84,33 @@ int main(int argc, char** argv) {
     for (RBFFD::DerType t : {RBFFD::X, RBFFD::Y, RBFFD::Z, RBFFD::LSFC})
         for (size_t s = 0; s < N; s++) std::fwrite(der.getStencilWeights(t, (int)s), 8, (size_t)n, o);
     std::fclose(o);
+    // weight files: X as MatrixMarket text (.mtx), Y as the binary form (.bmtx)
     der.setAsciiWeights(1);
     der.writeToFile(RBFFD::X, out + "_x.mtx");
+    der.setAsciiWeights(0);
+    der.writeToFile(RBFFD::Y, out + "_y.bmtx");
+    // read back into a second operator object on the same grid; its derivatives must equal the
+    // first object's bit for bit (%24.16le keeps 17 significant digits: exact for doubles)
+    RBFFD_HIP der2(RBFFD::X | RBFFD::Y, &grid, 3, 0);
+    der2.setAsciiWeights(1);
+    int rc_x = der2.loadFromFile(RBFFD::X, out + "_x.mtx");
+    der2.setAsciiWeights(0);
+    int rc_y = der2.loadFromFile(RBFFD::Y, out + "_y.bmtx");
+    int rc_missing = der2.loadFromFile(RBFFD::Y, out + "_missing.bmtx");
+    std::vector<double> dx2, dy2;
+    der2.applyWeightsForDeriv(RBFFD::X, u, dx2, true);
+    der2.applyWeightsForDeriv(RBFFD::Y, u, dy2, false);
+    bool reload_identical = dx2 == dx && dy2 == dy;
+    // the pointer overload writes deriv[i] for i in [start, start + nb) only
+    const unsigned start = (unsigned)(N / 3), nb = (unsigned)(N / 5);
+    std::vector<double> part(N, -7.0);
+    der.applyWeightsForDeriv(RBFFD::X, start, nb, u.data(), part.data(), true);
+    bool partial_ok = true;
+    for (size_t i = 0; i < N; i++) partial_ok &= (i >= start && i < start + nb) ? part[i] == dx[i] : part[i] == -7.0;
     std::printf("{\"N\": %zu, \"n\": %d, \"eps\": %g, \"poly_deg\": %d, \"max_backward_error\": %.3e, "
-                "\"rel_err_vs_analytic\": {\"x\": %.3e, \"y\": %.3e, \"z\": %.3e}}\n",
-                N, n, eps, deg, be, relmax(dx, fx), relmax(dy, fy), relmax(dz, fz));
+                "\"rel_err_vs_analytic\": {\"x\": %.3e, \"y\": %.3e, \"z\": %.3e}, "
+                "\"reload_identical\": %s, \"load_codes\": [%d, %d, %d], \"partial_apply_ok\": %s}\n",
+                N, n, eps, deg, be, relmax(dx, fx), relmax(dy, fy), relmax(dz, fz), reload_identical ? "true" : "false",
+                rc_x, rc_y, rc_missing, partial_ok ? "true" : "false");
     return 0;
 }

@@ -115,32 +115,60 @@ void Grid::computeStencilRadii() {
     }
 }
 
-void Grid::writeStencilsToFile(const std::string& filename) const {
-    std::ofstream f(filename);
-    for (const StencilType& st : stencil_map) {
-        for (size_t j = 0; j < st.size(); j++) f << st[j] << (j + 1 < st.size() ? " " : "");
-        f << "\n";
-    }
+std::string Grid::stencilFilename(const std::string& filename) const {
+    std::ostringstream prefix;
+    prefix << "stencils_maxsz" << max_st_size << "_" << filename;
+    return prefix.str();
 }
 
-int Grid::loadStencilsFromFile(const std::string& filename) {
-    std::ifstream f(filename);
-    if (!f.is_open()) return 1;
-    stencil_map.clear();
-    std::string line;
-    unsigned int mx = 0;
-    while (std::getline(f, line)) {
-        std::istringstream ss(line);
-        StencilType st;
-        int v;
-        while (ss >> v) st.push_back(v);
-        if (st.empty()) continue;
-        mx = st.size() > mx ? (unsigned)st.size() : mx;
-        stencil_map.push_back(st);
+void Grid::writeStencilsToFile(const std::string& filename) const {
+    if (max_st_size == 0) {
+        std::cout << "[Grid] \tMax stencil size not set. No stencils to write to disk" << std::endl;
+        return;
     }
-    max_st_size = mx;
+    const std::string fname = stencilFilename(filename);
+    std::ofstream fout(fname);
+    if (!fout.is_open()) {
+        std::printf("Error opening file to write\n");
+        std::exit(EXIT_FAILURE);
+    }
+    for (const StencilType& st : stencil_map) {
+        fout << st.size();
+        for (int v : st) fout << " " << v;
+        fout << "\n";
+    }
+    std::cout << "[Grid] \tWrote " << stencil_map.size() << " stencils to \t" << fname << std::endl;
+}
+
+Grid::GridLoadErrType Grid::loadStencilsFromFile(const std::string& filename) {
+    if (max_st_size == 0) {
+        std::cout << "[Grid] \tMax stencil size not set. No stencils to read from disk" << std::endl;
+        return GRID_AND_STENCILS_LOADED;
+    }
+    const std::string fname = stencilFilename(filename);
+    std::ifstream fin(fname);
+    if (!fin.is_open()) {
+        std::printf("Error opening stencil file to read\n");
+        return NO_STENCIL_FILES;
+    }
+    stencil_map.clear();
+    size_t loaded = 0;
+    // a record is "<size> i0 ... i(size-1)"; a record cut short by the end of the file is
+    // dropped, as the reference's loop drops it (it keeps a record only while !eof)
+    while (fin.good()) {
+        unsigned int st_size = 0;
+        if (!(fin >> st_size)) break;
+        StencilType st(st_size);
+        bool ok = true;
+        for (unsigned int i = 0; i < st_size && ok; i++) ok = static_cast<bool>(fin >> st[i]);
+        if (!ok) break;
+        stencil_map.push_back(st);
+        loaded += st.size();
+    }
     computeStencilRadii();
-    return 0;
+    std::cout << "[Grid] \tLoaded " << stencil_map.size() << " stencils, with a total of " << loaded
+              << " elements from \t" << fname << std::endl;
+    return GRID_AND_STENCILS_LOADED;
 }
 
 void GridReader::generate() { readNodeList(extra_); }
@@ -167,6 +195,11 @@ int GridReader::readNodeList(int expect_num_extra_dbls_per_line) {
         }
     }
     nb_nodes = (unsigned)node_list.size();
+    if (n_nodes_ != (unsigned)-1 && n_nodes_ > 0 && nb_nodes < n_nodes_) {
+        std::cout << "[GridReader] \tERROR: Found only " << nb_nodes << " nodes \t" << filename_
+                  << ". Check your configuration." << std::endl;
+        std::exit(EXIT_FAILURE);
+    }
     global_num_nodes = nb_nodes;
     return 0;
 }

@@ -29,6 +29,11 @@ class Grid {
     // ST_KDTREE_GPU (not in the reference): the kd-tree's lists from the GPU search
     // rbfx_knn_gpu, bit-identical to ST_KDTREE; it uses device 0 for the call.
     enum st_generator_t { ST_BRUTE_FORCE = 0, ST_KDTREE, ST_HASH, ST_KDTREE_GPU };
+    // Return codes of the loaders (grid_interface.h:38).
+    enum GridLoadErrType {
+        GRID_AND_STENCILS_LOADED = 0, NO_GRID_FILES, NO_BOUNDARY_FILES, NO_NORMAL_FILES, NO_STENCIL_FILES,
+        NO_RADII_FILES, NO_EXTRA_FILES, NOT_ENOUGH_NODES_IN_FILE
+    };
 
     Grid() {}
     explicit Grid(unsigned int num_nodes) : nb_nodes(num_nodes), global_num_nodes(num_nodes) {}
@@ -68,10 +73,15 @@ class Grid {
     double getMinStencilRadius(int i) const { return min_stencil_radii[i]; }
     double getMaxStencilRadius(int i) const { return max_stencil_radii[i]; }
 
-    // Stencil files in the reference's text layout (one stencil per line, indices separated by
-    // spaces; Grid::writeStencilsToFile / loadStencilsFromFile, grid_interface.cpp:144-170, 379-421).
+    // Stencil files in the reference's layout (Grid::writeStencilsToFile / loadStencilsFromFile,
+    // grid_interface.cpp:144-170, 379-421): the file is named "stencils_maxsz<max_st_size>_"
+    // followed by the grid file name, and each line is "<size> i0 i1 ..." (the centre first).
+    // Both need the maximum stencil size (setMaxStencilSize or generateStencils): without it
+    // nothing is written or read, as in the reference. loadStencilsFromFile returns
+    // NO_STENCIL_FILES when the file cannot be opened.
+    std::string stencilFilename(const std::string& filename) const;
     void writeStencilsToFile(const std::string& filename) const;
-    int loadStencilsFromFile(const std::string& filename);
+    GridLoadErrType loadStencilsFromFile(const std::string& filename);
 
   protected:
     void checkStencilSize();  // grid_interface.cpp:968-1002: clamps to N
@@ -85,7 +95,9 @@ class Grid {
 };
 
 // GridReader::readNodeList / generate (src/grids/grid_reader.cpp:36-124): whitespace-separated
-// "x y z [extra...]" lines; num_extra_cols extra doubles per line are skipped.
+// "x y z [extra...]" lines; num_extra_cols extra doubles per line are skipped. The constructor
+// only records the file; generate() reads it. With max_nodes set (not (unsigned)-1), at most
+// that many nodes are read and a file with fewer is an error (print + exit), as in the reference.
 class GridReader : public Grid {
   public:
     GridReader(const std::string& filename, int num_extra_cols = 0, unsigned int max_nodes = (unsigned)-1)

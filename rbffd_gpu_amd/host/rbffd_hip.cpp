@@ -128,41 +128,82 @@ void RBFFD::writeToFile(DerType which, const std::string& filename) {
 }
 
 int RBFFD::loadFromFile(DerType which, const std::string& filename) {
+    // RBFFD::loadFromAsciiFile / loadFromBinaryFile (rbffd.cpp:981-1160): the banner, the
+    // size line "M N nz", then nz (row, col, weight) triples, 1-based, as text or binary.
+    // Return codes as there: 1 no file, 2 bad banner, 4 bad size line, 5 M or nz differ from
+    // the grid's stencils; triples fill weights[row][j] in file order (j restarts at each new
+    // row), the column index is not used.
     FILE* f = std::fopen(filename.c_str(), "r");
-    if (!f) return 1;
+    if (!f) {
+        std::cout << "File not found: " << filename << std::endl;
+        return 1;
+    }
     char line[1024];
     if (!std::fgets(line, sizeof(line), f) || std::strncmp(line, "%%MatrixMarket", 14) != 0) {
+        std::cout << "Could not process MatrixMarket Banner in " << filename << std::endl;
         std::fclose(f);
         return 2;
     }
-    unsigned M = 0, N = 0;
-    unsigned long nz = 0;
+    long M = 0, N = 0, nz = 0;
     do {
-        if (!std::fgets(line, sizeof(line), f)) { std::fclose(f); return 3; }
+        if (!std::fgets(line, sizeof(line), f)) {
+            std::fclose(f);
+            return 4;
+        }
     } while (line[0] == '%');
-    if (std::sscanf(line, "%u %u %lu", &M, &N, &nz) != 3 || M != grid_ref.getStencilsSize()) {
+    if (std::sscanf(line, "%ld %ld %ld", &M, &N, &nz) != 3) {
+        std::cout << "Error! failed to parse file contents" << std::endl;
         std::fclose(f);
-        return 3;
+        return 4;
     }
     std::vector<StencilType>& st = grid_ref.getStencils();
     std::vector<double*>& w = weights[getDerTypeIndx(which)];
-    for (size_t i = 0; i < st.size(); i++) {
-        delete[] w[i];
-        w[i] = new double[st[i].size()];
-        for (size_t j = 0; j < st[i].size(); j++) {
-            int ii, jj;
-            double kk;
-            bool ok = asciiWeights ? std::fscanf(f, "%d %d %lf", &ii, &jj, &kk) == 3
-                                   : (std::fread(&ii, sizeof(int), 1, f) == 1 && std::fread(&jj, sizeof(int), 1, f) == 1 &&
-                                      std::fread(&kk, sizeof(double), 1, f) == 1);
-            if (!ok || jj != st[i][j] + 1) {
-                std::fclose(f);
-                return 4;
-            }
-            w[i][j] = kk;
+    long expected_nz = 0;
+    for (auto& s : st) expected_nz += (long)s.size();
+    if (M != (long)w.size()) {
+        std::cout << "Error! not enough stencils in the file" << std::endl;
+        std::fclose(f);
+        return 5;
+    }
+    if (nz != expected_nz) {
+        std::cout << "Error! not enough weights in the file" << std::endl;
+        std::fclose(f);
+        return 5;
+    }
+    std::vector<int> I(nz);
+    std::vector<double> val(nz);
+    for (long q = 0; q < nz; q++) {
+        int ii = 0, jj = 0;
+        double kk = 0.0;
+        bool ok = asciiWeights ? std::fscanf(f, "%d %d %le", &ii, &jj, &kk) == 3
+                               : (std::fread(&ii, sizeof(int), 1, f) == 1 && std::fread(&jj, sizeof(int), 1, f) == 1 &&
+                                  std::fread(&kk, sizeof(double), 1, f) == 1);
+        if (!ok || ii < 1 || ii > M) {
+            std::cout << "Error! failed to parse file contents" << std::endl;
+            std::fclose(f);
+            return 4;
         }
+        I[q] = ii - 1;
+        val[q] = kk;
     }
     std::fclose(f);
+    for (size_t i = 0; i < st.size(); i++) {
+        delete[] w[i];
+        w[i] = new double[st[i].size()]();
+    }
+    long j = 0;
+    int row = 0;
+    for (long q = 0; q < nz; q++) {
+        if (I[q] != row) {
+            j = 0;
+            row = I[q];
+        }
+        if (j >= (long)st[row].size()) {
+            std::printf("[RBFFD] ERROR! File contains too many nonzeros. I have not allocated enough memory!\n");
+            std::exit(EXIT_FAILURE);
+        }
+        w[row][j++] = val[q];
+    }
     computedTypes |= which;
     weightsModified = true;
     return 0;
@@ -369,7 +410,8 @@ void RBFFD_HIP::applyWeightsForDeriv(DerType which, std::vector<double>& u, std:
     applyWeightsForDeriv(which, 0, nb_stencils, &u[0], &deriv[0], isChangedU);
 }
 
-// All rows are applied on the device (as RBFFD_CL does); rows [start, start+nb) are returned.
+// All rows are applied on the device (as RBFFD_CL does); rows [start, start+nb) are returned in
+// deriv[start .. start+nb), the entries the reference's loop writes (rbffd.cpp:669-688).
 void RBFFD_HIP::applyWeightsForDeriv(DerType which, unsigned int start_indx, unsigned int nb_stencils, double* u,
                                      double* deriv, bool isChangedU) {
     rbfx_ell* e = getGPUWeights(which);
@@ -379,12 +421,17 @@ void RBFFD_HIP::applyWeightsForDeriv(DerType which, unsigned int start_indx, uns
     int ft = storage == RBFX_F64 ? RBFX_F64 : RBFX_F32;
     int ac = storage == RBFX_F64 ? RBFX_F64 : accum;
     RBFX_CALL(ctx, rbfx_spmv(e, 0, d_u, d_out, ft, ac));
+    if (start_indx + (size_t)nb_stencils > S) {
+        std::cout << "[RBFFD_HIP] ERROR: applyWeightsForDeriv rows [" << start_indx << ", "
+                  << start_indx + (size_t)nb_stencils << ") exceed the " << S << " stencils" << std::endl;
+        std::exit(EXIT_FAILURE);
+    }
     if (ft == RBFX_F64) {
-        RBFX_CALL(ctx, rbfx_d2h(ctx, deriv, (double*)d_out + start_indx, (size_t)nb_stencils * 8));
+        RBFX_CALL(ctx, rbfx_d2h(ctx, deriv + start_indx, (double*)d_out + start_indx, (size_t)nb_stencils * 8));
     } else {
         std::vector<float> f(nb_stencils);
         RBFX_CALL(ctx, rbfx_d2h(ctx, f.data(), (float*)d_out + start_indx, (size_t)nb_stencils * 4));
-        for (unsigned i = 0; i < nb_stencils; i++) deriv[i] = f[i];
+        for (unsigned i = 0; i < nb_stencils; i++) deriv[start_indx + i] = f[i];
     }
 }
 

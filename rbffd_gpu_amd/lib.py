@@ -72,6 +72,7 @@ def load_library(path=None):
         "rbfx_nodes_fibonacci": (C.c_int, [i32, f64, u64, vp]),
         "rbfx_nodes_icosahedral": (C.c_int, [i32, vp, C.POINTER(i32)]),
         "rbfx_order_morton": (C.c_int, [i32, vp, vp]),
+        "rbfx_sph_harm": (C.c_int, [i32, i32, i32, vp, vp]),
         "rbfx_ell_set_row_order": (C.c_int, [vp, vp]),
         "rbfx_ell_build_local": (C.c_int, [vp, i32, i32]),
         "rbfx_ell_local_info": (C.c_int, [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)]),
@@ -179,6 +180,32 @@ def order_morton(xyz):
     if rc:
         raise RbfxError(rc, "order_morton failed")
     return out
+
+
+def sph_harm(l, m, xyz):
+    """Real spherical harmonic Y_lm at the directions of xyz (N x 3), computed in librbfx."""
+    xyz = np.ascontiguousarray(xyz, np.float64)
+    out = np.empty(xyz.shape[0])
+    rc = RBFX.rbfx_sph_harm(int(l), int(m), xyz.shape[0], _ptr(xyz), _ptr(out))
+    if rc:
+        raise RbfxError(rc, f"sph_harm({l}, {m}) failed")
+    return out
+
+
+def make_fields(xyz, F, seed=0, lmax=20, terms=6):
+    """F seeded fields, each a combination of `terms` real Y_lm with l <= lmax and standard
+    normal coefficients; returns (N x F array, [(l, m, c), ...] per field)."""
+    rng = np.random.default_rng(seed)
+    spec = []
+    u = np.zeros((np.asarray(xyz).shape[0], F))
+    for f in range(F):
+        ls = rng.integers(0, lmax + 1, size=terms)
+        ms = [int(rng.integers(-l, l + 1)) for l in ls]
+        cs = rng.standard_normal(terms)
+        spec.append(list(zip(ls.tolist(), ms, cs.tolist())))
+        for l, m, c in spec[-1]:
+            u[:, f] += c * sph_harm(int(l), int(m), xyz)
+    return u, spec
 
 
 def csr_from_fixed(st):

@@ -14,6 +14,18 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP kernels through the C ABI)")
 
 
+@pytest.fixture(autouse=True)
+def _fresh_tuning_table():
+    """Each test starts and ends with an empty launch-shape table (it is process-global)."""
+    import rbffd_gpu_amd as R
+
+    yield
+    try:
+        R.lib.RBFX.rbfx_tune_clear()
+    except OSError:
+        pass
+
+
 @pytest.fixture(scope="session")
 def ctx():
     import rbffd_gpu_amd as R

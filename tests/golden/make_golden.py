@@ -16,11 +16,22 @@ Fixtures (data only):
   lapack_weights.npz  stencil systems solved by scipy.linalg.solve (LAPACK getrf/getrs,
                    the routine arma::solve reaches): node coordinates, eps, mask, degree,
                    the solution and cond(M)
+  regulargrid_10x_10y_1z_final.ascii   the reference's own node fixture (a data file of
+                   its tests/unified/), copied unchanged
+  stencils_maxsz<k>_regulargrid_10x_10y_1z_final.ascii   the stencil files the reference's
+                   Grid::writeStencilsToFile writes for that grid (k = 9, 13), produced by
+                   oracle/_ref/ref_knn from the compiled reference sources
+
+`python tests/golden/make_golden.py stencil_files` rewrites only the last two.
 """
 import hashlib
 import json
 import os
+import shutil
+import struct
+import subprocess
 import sys
+import tempfile
 
 import numpy as np
 
@@ -39,8 +50,27 @@ def sha(a):
     return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
 
 
+def stencil_files():
+    """The reference node fixture and the reference's own stencil files for it."""
+    name = os.path.basename(REF_GRID)
+    shutil.copyfile(REF_GRID, os.path.join(HERE, name))
+    grid = np.loadtxt(REF_GRID)[:, :3]
+    with tempfile.TemporaryDirectory() as td:
+        fn = os.path.join(td, "n.bin")
+        with open(fn, "wb") as f:
+            f.write(struct.pack("<q", grid.shape[0]))
+            f.write(grid.tobytes())
+        for k in (9, 13):
+            subprocess.run([O.REF_KNN, fn, str(k), os.path.join(td, "o.bin"), name], cwd=td, check=True,
+                           stdout=subprocess.DEVNULL)
+            out = f"stencils_maxsz{k}_{name}"
+            shutil.copyfile(os.path.join(td, out), os.path.join(HERE, out))
+
+
 def main():
     import scipy.linalg as sl
+
+    stencil_files()
 
     # ---- config 1 -----------------------------------------------------------------------
     xyz = R.nodes_fibonacci(4096, 1e-3, 0)
@@ -100,4 +130,7 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["stencil_files"]:
+        stencil_files()
+    else:
+        main()

@@ -55,6 +55,23 @@ def test_derivative_driver_c1(tmp_path, c1):
     assert mtx[1].split() == ["4096", "4096", str(4096 * 31)]
     r0 = mtx[2].split()
     assert int(r0[0]) == 1 and int(r0[1]) == st[0, 0] + 1 and float(r0[2]) == w_gpu[0, 0, 0]
+    # every entry of the text file, and of the binary file (banner, size line, then int32 row,
+    # int32 col, float64 weight records; rbffd.cpp:1302-1375)
+    rows = np.array([line.split() for line in mtx[2:]])
+    assert np.array_equal(rows[:, 0].astype(int), np.repeat(st[:, 0], 31) + 1)
+    assert np.array_equal(rows[:, 1].astype(int), st.reshape(-1) + 1)
+    assert np.array_equal(rows[:, 2].astype(float), w_gpu[0].reshape(-1))
+    raw = open(tmp_path / "c1_y.bmtx", "rb").read()
+    head = raw.split(b"\n", 2)
+    assert head[0].startswith(b"%%MatrixMarket") and head[1].split() == [b"4096", b"4096", str(4096 * 31).encode()]
+    rec = np.frombuffer(head[2], dtype=np.dtype([("i", "<i4"), ("j", "<i4"), ("w", "<f8")]))
+    assert rec.size == 4096 * 31
+    assert np.array_equal(rec["i"], np.repeat(st[:, 0], 31) + 1) and np.array_equal(rec["j"], st.reshape(-1) + 1)
+    assert np.array_equal(rec["w"], w_gpu[1].reshape(-1))
+    # loadFromFile reads both back (the re-applied derivatives are bit-identical), reports a
+    # missing file with code 1, and the pointer overload writes only rows [start, start + nb)
+    assert res["reload_identical"] and res["load_codes"] == [0, 0, 1]
+    assert res["partial_apply_ok"]
     print("analytic errors:", res["rel_err_vs_analytic"])
 
 

@@ -162,7 +162,6 @@ def test_five_point_laplacian_flat_limit_kat():
 
 def test_laplace_beltrami_on_spherical_harmonics():
     """Delta_S Y_lm = -l(l+1) Y_lm on nodes exactly on the unit sphere (icosahedral)."""
-    from bench import real_sph_harm
 
     xyz = R.nodes_icosahedral(4)
     n = 31
@@ -171,7 +170,7 @@ def test_laplace_beltrami_on_spherical_harmonics():
     N = xyz.shape[0]
     w, _, _ = O.weights(off, idx, xyz, 0.035 * np.sqrt(N) - 0.1, R.LSFC, dim=2, poly_deg=0)
     for l, m in ((1, 0), (2, 1), (3, -2), (4, 3)):
-        Y = real_sph_harm(l, m, xyz)
+        Y = R.sph_harm(l, m, xyz)
         d = O.apply(off, idx, w[0], Y)
         err = np.abs(d + l * (l + 1) * Y).max() / (l * (l + 1) * np.abs(Y).max())
         assert err < 2e-3, (l, m, err)
@@ -263,7 +262,6 @@ def test_norms_against_numpy():
 def test_planted_mistake_is_caught():
     """A dropped term in the restated RHS (here: the LSFC phi'' term) must fail the
     spherical-harmonic check above: the check is sensitive to the formula."""
-    from bench import real_sph_harm
 
     xyz = R.nodes_icosahedral(4)
     st = R.knn_kdtree(xyz, 31)
@@ -273,7 +271,7 @@ def test_planted_mistake_is_caught():
     # LSFC restated without its phi'' term equals 1/4 (4 - 3 r^2) phi'/r: build that "wrong"
     # operator from the XSFC-independent pieces by solving with a modified right-hand side
     M_ok = []
-    Y = real_sph_harm(2, 1, xyz)
+    Y = R.sph_harm(2, 1, xyz)
     good = O.apply(off, idx, O.weights(off, idx, xyz, eps, R.LSFC, dim=2)[0][0], Y)
     wrong = np.empty(N)
     for i in range(N):
@@ -293,7 +291,6 @@ def test_sphere_reduced_basis_reproduces_degree2_harmonics():
     """On nodes exactly on the sphere the full degree-2 basis is singular; the 9-function
     basis is not, and Laplace-Beltrami weights then reproduce Delta_S Y_lm = -l(l+1) Y_lm for
     l <= 2 to rounding level (polynomial reproduction)."""
-    from bench import real_sph_harm
 
     xyz = R.nodes_icosahedral(4)
     st = R.knn_kdtree(xyz, 31)
@@ -308,7 +305,7 @@ def test_sphere_reduced_basis_reproduces_degree2_harmonics():
     w, _, ns = O.weights(off, idx, xyz, 0.035 * np.sqrt(N) - 0.1, R.LSFC, dim=2, poly_deg=2 | R.POLY_SPHERE)
     assert ns == 0
     for l, m in ((1, 0), (2, -1), (2, 2)):
-        Y = real_sph_harm(l, m, xyz)
+        Y = R.sph_harm(l, m, xyz)
         d = O.apply(off, idx, w[0], Y)
         assert np.abs(d + l * (l + 1) * Y).max() < 1e-6 * l * (l + 1), (l, m)
 
@@ -341,3 +338,138 @@ def test_round_f32_rowsum_properties():
         bad = O.round_f32_rowsum(Wt).astype(np.float64)
         bad[:, 1] += 2 * np.spacing(np.abs(Wt[:, 1]).astype(np.float32))
         assert np.any(np.abs(bad - Wt) > 1.5 * np.spacing(np.abs(Wt.astype(np.float32))).astype(np.float64))
+
+
+# ---- degree-3 augmentation, multi-operator RK4, cosine-bell right-hand side -------------------
+def _cubics(xyz):
+    """Every monomial x^a y^b z^c with a + b + c <= 3 in absolute coordinates, with its exact
+    gradient and Laplacian (computed here from the exponents, not from the oracle's tables)."""
+    x, y, z = xyz.T
+    out = []
+    for a in range(4):
+        for b in range(4 - a):
+            for c in range(4 - a - b):
+                def mono(p, q, r):
+                    if p < 0 or q < 0 or r < 0:
+                        return np.zeros_like(x)
+                    return x ** p * y ** q * z ** r
+                val = mono(a, b, c)
+                grad = [a * mono(a - 1, b, c), b * mono(a, b - 1, c), c * mono(a, b, c - 1)]
+                lap = a * (a - 1) * mono(a - 2, b, c) + b * (b - 1) * mono(a, b - 2, c) + c * (c - 1) * mono(a, b, c - 2)
+                out.append(((a, b, c), val, grad, lap))
+    return out
+
+
+def test_polynomial_reproduction_degree3_full_basis():
+    """Degree-3 augmentation (20 monomials): X/Y/Z/LAPL weights differentiate every cubic
+    exactly (up to rounding times the conditioning), in absolute coordinates. A dropped or
+    duplicated cubic monomial in the basis fails this (a missing one is not reproduced, a
+    duplicate makes the system singular)."""
+    xyz = R.nodes_fibonacci(3000, 1e-3, 4)
+    st = R.knn_kdtree(xyz, 50)
+    off, idx = R.csr_from_fixed(st)
+    assert O.lib().orc_num_monomials(3) == 20
+    w, _, ns = O.weights(off, idx, xyz, 4.0, R.X | R.Y | R.Z | R.LAPL, dim=3, poly_deg=3)
+    assert ns == 0
+    w = w.reshape(4, 3000, 50)
+    polys = _cubics(xyz)
+    assert len(polys) == 20
+    for i in (0, 1234, 2999):
+        for (abc, val, grad, lap) in polys:
+            want = [grad[0][i], grad[1][i], grad[2][i], lap[i]]
+            for t in range(4):
+                got = w[t, i] @ val[st[i]]
+                scale = np.abs(w[t, i]).sum() * np.abs(val[st[i]]).max() + 1.0
+                assert abs(got - want[t]) <= 1e-8 * scale, (i, abc, t, got, want[t])
+
+
+def test_polynomial_reproduction_degree3_sphere_basis():
+    """Sphere-reduced degree-3 basis (16 functions) on nodes exactly on the unit sphere:
+    Laplace-Beltrami weights reproduce Delta_S p = -l(l+1) p for the harmonic homogeneous
+    polynomials of degree l <= 3 (their restrictions are spherical harmonics), and surface
+    gradient weights reproduce grad_S p = grad p - l p x (Euler's relation x.grad p = l p)."""
+    xyz = R.nodes_icosahedral(4)
+    st = R.knn_kdtree(xyz, 50)
+    off, idx = R.csr_from_fixed(st)
+    N = xyz.shape[0]
+    w, _, ns = O.weights(off, idx, xyz, 4 * (0.044 * np.sqrt(N) - 0.14), R.LSFC | R.XSFC | R.YSFC | R.ZSFC, dim=2,
+                         poly_deg=3 | R.POLY_SPHERE)
+    assert ns == 0
+    x, y, z = xyz.T
+    harm = [  # (degree, value, gradient)
+        (0, np.ones_like(x), [0 * x, 0 * x, 0 * x]),
+        (1, z, [0 * x, 0 * x, 1 + 0 * x]),
+        (2, x * y, [y, x, 0 * x]),
+        (2, x * x - y * y, [2 * x, -2 * y, 0 * x]),
+        (3, x * y * z, [y * z, x * z, x * y]),
+        (3, x ** 3 - 3 * x * y * y, [3 * x * x - 3 * y * y, -6 * x * y, 0 * x]),
+        (3, 2 * z ** 3 - 3 * x * x * z - 3 * y * y * z, [-6 * x * z, -6 * y * z, 6 * z * z - 3 * x * x - 3 * y * y]),
+    ]
+    for l, p, g in harm:
+        lb = O.apply(off, idx, w[0], p)
+        assert np.abs(lb + l * (l + 1) * p).max() < 1e-7 * max(1, l * (l + 1)), (l, np.abs(lb + l * (l + 1) * p).max())
+        for c in range(3):
+            want = g[c] - l * p * xyz[:, c]
+            assert np.abs(O.apply(off, idx, w[1 + c], p) - want).max() < 1e-7, (l, c)
+    # degree 2 alone does not reproduce the cubic harmonics: the test separates the degrees
+    w2, _, _ = O.weights(off, idx, xyz, 4 * (0.044 * np.sqrt(N) - 0.14), R.LSFC, dim=2, poly_deg=2 | R.POLY_SPHERE)
+    p = x * y * z
+    assert np.abs(O.apply(off, idx, w2[0], p) + 12 * p).max() > 1e-6
+
+
+def test_rk4_multi_operator_solid_body_rotation():
+    """u_t = y D_x u - x D_y u (two operators with coefficient arrays, two fields) is rotation
+    about the z axis: u(t, x, y, z) = u0(x cos t + y sin t, -x sin t + y cos t, z). With
+    degree-2 weights the operators reproduce the quadratic fields xz and x^2 - y^2 up to the
+    rounding of the weights (about kappa u, 1e-6 here), and RK4 adds its O(dt^4) truncation
+    error; halving dt changes the result by less than that bound. Swapping the coefficients
+    between the operators, dropping one, or flipping a sign gives O(1) errors."""
+    xyz = R.nodes_fibonacci(2000, 1e-3, 6)
+    st = R.knn_kdtree(xyz, 31)
+    off, idx = R.csr_from_fixed(st)
+    w, _, ns = O.weights(off, idx, xyz, 3.0, R.X | R.Y, dim=3, poly_deg=2)
+    assert ns == 0
+    x, y, z = xyz.T
+
+    def exact(t):
+        xr, yr = x * np.cos(t) + y * np.sin(t), -x * np.sin(t) + y * np.cos(t)
+        return np.column_stack([xr * z, xr * xr - yr * yr])
+
+    T, steps = 1.0, 100
+    u = O.rk4_step(st, [w[0], w[1]], [y, -x], exact(0.0), T / steps, n_steps=steps)
+    err = np.abs(u - exact(T)).max()
+    assert err < 2e-5, err
+    u2 = O.rk4_step(st, [w[0], w[1]], [y, -x], exact(0.0), T / (2 * steps), n_steps=2 * steps)
+    assert np.abs(u2 - u).max() < 1e-6  # RK4 truncation at dt = 0.01: about 3e-7
+    for ws, cs in (([w[0], w[1]], [-x, y]), ([w[0]], [y]), ([w[0], w[1]], [y, x])):
+        with np.errstate(all="ignore"):
+            bad = O.rk4_step(st, ws, cs, exact(0.0), T / steps, n_steps=steps)
+            assert not np.all(np.abs(bad - exact(T)) <= 1e-2)  # also when the wrong system blows up
+
+
+def test_cosine_bell_rhs_of_a_smooth_field_is_minus_u_grad_h():
+    """CosineBell::solve without hyperviscosity on a smooth field h equals the analytic
+    -u.grad h of the solid-body velocity (alpha = pi/2: rotation about the x axis,
+    u = Omega x x with Omega = (-u0/a, 0, 0)) to discretisation error."""
+    xyz = R.nodes_icosahedral(4)
+    st = R.knn_kdtree(xyz, 31)
+    off, idx = R.csr_from_fixed(st)
+    N = xyz.shape[0]
+    eps = 0.035 * np.sqrt(N) - 0.1
+    w, _, _ = O.weights(off, idx, xyz, eps, R.LAMBDA | R.THETA | R.HV, dim=2, poly_deg=0, hv_k=4,
+                        hv_scale=-0.05 * N ** -4.0)
+    x, y, z = xyz.T
+    h = np.exp(x) * np.sin(2 * y) + z ** 3
+    grad = np.column_stack([np.exp(x) * np.sin(2 * y), 2 * np.exp(x) * np.cos(2 * y), 3 * z * z])
+    T = 1036800.0
+    u0 = 2 * np.pi / T
+    omega = np.array([-u0, 0.0, 0.0])
+    vel = np.cross(omega, xyz)
+    want = -(vel * grad).sum(1)
+    # weight columns in ascending DerType bit order: HV (0x20), LAMBDA (0x40), THETA (0x80)
+    got = O.cosine_bell_rhs(off, idx, w[1], w[2], w[0], xyz, h, use_hv=0)
+    err = np.abs(got - want).max() / np.abs(want).max()
+    assert err < 5e-3, err
+    # the velocity's sign or the 1/cos(theta) of the zonal term matter
+    wrong = O.cosine_bell_rhs(off, idx, w[1], w[2], w[0], xyz, h, alpha=-np.pi / 2, use_hv=0)
+    assert np.abs(wrong - want).max() / np.abs(want).max() > 0.5
