@@ -386,38 +386,53 @@ __device__ void forward_solve(const Sys& S, TA* Aug, const int (&posr)[2]) {
 
 // r = b - M x in FP64 with M rebuilt from the coordinates; x(k,t) = XS[k*T+t] (by unknown).
 // With Aug set, r is written into its RHS columns (as TA). Returns the normwise backward
-// error max_t ||r_t|| / (||M|| ||x_t|| + ||b_t||) (max norms, ||M|| = max row sum).
+// error max_t ||r_t|| / (||M|| ||x_t|| + ||b_t||) (max norms, ||M|| = max row sum). The
+// right-hand sides are taken RC at a time (M is rebuilt once per group) so that the
+// accumulators stay in registers with compile-time indices; the operations of each r_t are
+// the same in any grouping.
 template <typename TA, int MT>
 __device__ double residual(const Sys& S, TA* Aug, const double* XS) {
-    const int lane = threadIdx.x, m = S.m, ld = S.ld, T = S.T;
-    double rmax[MT], xmax[MT], bmax[MT];
-    for (int t = 0; t < T; t++) rmax[t] = xmax[t] = bmax[t] = 0.0;
-    double rowsum = 0.0;
-    for (int h = 0; h < 2; h++) {
-        const int r = lane + h * WAVE;
-        if (r >= m) continue;
-        double acc[MT];
-        for (int t = 0; t < T; t++) {
-            double b = S.BS ? S.BS[r * T + t] : rhs_entry(S, r, t);
-            acc[t] = b;
-            bmax[t] = fmax(bmax[t], fabs(b));
-        }
-        double rs = 0.0;
-        for (int c = 0; c < m; c++) {
-            double a = m_entry(r, c, S.n, S.e2, S.X, S.P->sph);
-            rs += fabs(a);
-            for (int t = 0; t < T; t++) acc[t] -= a * XS[c * T + t];
-        }
-        rowsum = fmax(rowsum, rs);
-        for (int t = 0; t < T; t++) {
-            rmax[t] = fmax(rmax[t], fabs(acc[t]));
-            xmax[t] = fmax(xmax[t], fabs(XS[r * T + t]));
-            if (Aug) Aug[r * ld + m + t] = (TA)acc[t];
-        }
-    }
-    rowsum = wave_max(rowsum);
+    constexpr int RC = 4;
+    const int lane = threadIdx.x & (WAVE - 1), m = S.m, ld = S.ld, T = S.T;
     double be = 0.0;
-    for (int t = 0; t < T; t++) be = fmax(be, wave_max(rmax[t]) / (rowsum * wave_max(xmax[t]) + wave_max(bmax[t])));
+    for (int t0 = 0; t0 < T; t0 += RC) {
+        double rmax[RC], xmax[RC], bmax[RC];
+#pragma unroll
+        for (int t = 0; t < RC; t++) rmax[t] = xmax[t] = bmax[t] = 0.0;
+        double rowsum = 0.0;
+        for (int h = 0; h < 2; h++) {
+            const int r = lane + h * WAVE;
+            if (r >= m) continue;
+            double acc[RC];
+#pragma unroll
+            for (int t = 0; t < RC; t++) {
+                double b = 0.0;
+                if (t0 + t < T) b = S.BS ? S.BS[r * T + t0 + t] : rhs_entry(S, r, t0 + t);
+                acc[t] = b;
+                bmax[t] = fmax(bmax[t], fabs(b));
+            }
+            double rs = 0.0;
+            for (int c = 0; c < m; c++) {
+                double a = m_entry(r, c, S.n, S.e2, S.X, S.P->sph);
+                rs += fabs(a);
+#pragma unroll
+                for (int t = 0; t < RC; t++)
+                    if (t0 + t < T) acc[t] -= a * XS[c * T + t0 + t];
+            }
+            rowsum = fmax(rowsum, rs);
+#pragma unroll
+            for (int t = 0; t < RC; t++) {
+                if (t0 + t >= T) continue;
+                rmax[t] = fmax(rmax[t], fabs(acc[t]));
+                xmax[t] = fmax(xmax[t], fabs(XS[r * T + t0 + t]));
+                if (Aug) Aug[r * ld + m + t0 + t] = (TA)acc[t];
+            }
+        }
+        rowsum = wave_max(rowsum);
+#pragma unroll
+        for (int t = 0; t < RC; t++)
+            if (t0 + t < T) be = fmax(be, wave_max(rmax[t]) / (rowsum * wave_max(xmax[t]) + wave_max(bmax[t])));
+    }
     return be;
 }
 
@@ -1029,11 +1044,512 @@ size_t f64_reg_lds(int m, int n, int T) {
     return reg_lds_bytes(m, n, T, cols <= 96 ? 96 : cols <= 128 ? 128 : 192, 2 * WAVE);
 }
 
-// RBFX_WEIGHTS_F64=lds (read at each call) selects the one-wave LDS kernel for FP64 solves;
-// it gives the same weights bit for bit.
-bool f64_use_lds() {
-    const char* e = std::getenv("RBFX_WEIGHTS_F64");
-    return e && std::strcmp(e, "lds") == 0;
+// ---- persistent blocked LU on the matrix cores (FP64 default) -----------------------------
+// One launch solves every stencil: LU_BLOCK-thread workgroups stay resident and take work
+// units (one large stencil, or one stencil per wave for small systems; largest systems first)
+// from a device counter. A stencil's [M | R] lives in registers as 16x16 tiles in the
+// accumulator layout of v_mfma_f64_16x16x4f64, transposed: register v of lane l of a tile
+// holds row l%16, column 4v + l/16. Wave w of a W-wave team owns the
+// column tiles w, w + W, ... (NCT of them); RT row tiles cover the m rows.
+// Partial-pivot LU in panels of 4 columns (the K of the instruction):
+//   1. the owner wave of the panel's column tile copies the panel to LDS and factors it
+//      row-per-lane: per column a pivot search over the unpivoted rows (the largest |a|,
+//      ties to the smaller row, as in the unblocked kernels), multipliers, and the update of
+//      the panel's remaining columns; the row order is kept in perm/pos instead of moving rows;
+//   2. it stores the multipliers as the B operand L[r][q] (zero for rows pivoted earlier; a
+//      row that became the pivot of column q keeps its multipliers of columns < q);
+//   3. every wave forms the panel's four U rows on its own columns (the pivot rows' values
+//      with the unit-lower 4x4 block applied), zero on finished columns, as the A operand;
+//   4. every wave updates its tiles with one v_mfma_f64_16x16x4f64 per (row tile, live
+//      column tile): D = C - U12^T L21^T, i.e. a[r][c] -= sum_q L[r][q] U[q][c].
+// The instruction computes D[i][j] = C[i][j] + sum_k A[i][k] B[k][j] with A[i][k] from lane
+// i + 16k, B[k][j] from lane j + 16k and D[i][j] in register i/4 of lane j + 16(i%4), and
+// accumulates the four products in order, each fused and rounded on its own (a probe of all
+// 256 outputs on random operands equals four chained fma). Here i is the column and j the
+// row: A[i][k] = -U[k][c], B[k][j] = L[r][k]. So every element
+// sees the operations of the unblocked register kernel in the same order and the weights are
+// bit-identical to it. The pivot rows' own updates in step 4 reproduce the U rows of step 3.
+// After the last panel each lane stores its rows' U part and eliminated right-hand sides to
+// LDS by pivot order, and back substitution runs as in the register kernel.
+typedef double d4v __attribute__((ext_vector_type(4)));
+
+constexpr int LU_BLOCK = 256;   // threads per workgroup (4 waves)
+constexpr int LU_WAVES = LU_BLOCK / WAVE;
+constexpr int LU_NLAYOUT = 6;   // layout 5 (three column tiles per wave) only in the wide kernel
+
+struct LuLayout {
+    int W, RT, NCT;
+};
+
+// W waves per stencil, RT row tiles, NCT column tiles per wave; a workgroup runs
+// LU_WAVES / W stencils of one size at a time.
+__host__ __device__ constexpr LuLayout lu_layout(int id) {
+    return id == 0 ? LuLayout{1, 2, 2}
+         : id == 1 ? LuLayout{2, 3, 2}
+         : id == 2 ? LuLayout{2, 4, 2}
+         : id == 3 ? LuLayout{4, 6, 2}
+         : id == 4 ? LuLayout{4, 8, 2}
+                   : LuLayout{4, 8, 3};
+}
+
+// Smallest layout holding m rows and m + T columns; -1 when none does.
+__host__ __device__ inline int lu_pick(int m, int T) {
+    for (int id = 0; id < LU_NLAYOUT; id++) {
+        const LuLayout L = lu_layout(id);
+        if (m <= 16 * L.RT && m + T <= 16 * L.W * L.NCT) return id;
+    }
+    return -1;
+}
+
+// LDS of one team, in doubles: [R0][BS: m T][dg: m][amax: 8][umax: 8] then ints
+// [perm: m][pos: 16 RT][flag: 4]. R0 holds the staged coordinates (5n) during assembly, the
+// panel, two multiplier buffers and the U-row buffer (12 * 16 RT + 4 * 16 W NCT) during the
+// factorisation, and the strict upper triangle by columns (m(m-1)/2) for back substitution.
+__host__ __device__ inline size_t lu_r0(int W, int RT, int NCT, int m, int n) {
+    const size_t a = (size_t)5 * n, b = (size_t)12 * 16 * RT + (size_t)4 * 16 * W * NCT, c = (size_t)m * (m - 1) / 2;
+    const size_t ab = a > b ? a : b;
+    return ab > c ? ab : c;
+}
+__host__ __device__ inline size_t lu_team_bytes(int W, int RT, int NCT, int m, int n, int T) {
+    const size_t d = lu_r0(W, RT, NCT, m, n) + (size_t)m * T + m + 16;
+    const size_t i = (size_t)m + 16 * RT + 4;
+    return (d * sizeof(double) + i * sizeof(int) + 15) / 16 * 16;
+}
+
+// The value of x, hidden from the optimiser: lane-dependent indices derived from it are
+// recomputed where they are used instead of being hoisted out of the persistent and panel
+// loops, where dozens of them would stay live in VGPRs next to the matrix tiles.
+__device__ __forceinline__ int opaque_v(int x) {
+    asm volatile("" : "+v"(x));
+    return x;
+}
+
+template <int W>
+__device__ __forceinline__ void team_sync() {
+    if constexpr (W == 1)
+        wave_lds_sync();
+    else
+        __syncthreads();
+}
+
+// Solves the stencils order[start .. start + count) (count <= number of teams), each of n
+// nodes and m = n + np unknowns, one per team. One-wave teams without a stencil return at
+// once; larger teams without one repeat the first team's solve and write nothing, so that
+// every team passes the same workgroup barriers.
+template <int W, int RT, int NCT>
+__device__ __forceinline__ void lu_team(const WParams& P, int start, int count, int n, int m, unsigned char* smem) {
+    constexpr int MP = 16 * RT, NP = 16 * W * NCT, RR = (MP + WAVE - 1) / WAVE;
+    const int wv = threadIdx.x / WAVE;
+    int lane = opaque_v(threadIdx.x & (WAVE - 1));
+    const int team = wv / W, wt = wv - team * W;
+    if (W == 1 && team >= count) return;
+    bool writer = team < count;
+    const int s = P.order[start + (writer ? team : 0)];
+    const int T = P.T;
+    int rl = lane & 15, cg = lane >> 4;
+    Sys S;
+    S.P = &P;
+    S.s = s;
+    S.off = P.st_off[s];
+    S.n = n;
+    S.m = m;
+    S.T = T;
+    S.ld = 0;
+    S.eps = P.eps[s];
+    S.e2 = S.eps * S.eps;
+    S.sph = false;
+    for (int t = 0; t < T; t++) S.sph |= (P.types[t] == RBFX_LAMBDA || P.types[t] == RBFX_THETA);
+    S.XS = S.BS = nullptr;
+    S.perm = nullptr;
+    // team LDS
+    unsigned char* base = smem + (size_t)team * lu_team_bytes(W, RT, NCT, m, n, T);
+    double* R0 = (double*)base;
+    double* BS = R0 + lu_r0(W, RT, NCT, m, n);
+    double* dg = BS + (size_t)m * T;
+    double* amaxw = dg + m;
+    double* umaxw = amaxw + 8;
+    int* perm = (int*)(umaxw + 8);
+    int* pos = perm + m;
+    int* flag = pos + MP;
+    double* Pbuf = R0;                  // [MP][4]
+    double* Lbuf = R0 + MP * 4;         // [2][MP][4]
+    double* U12 = R0 + 3 * MP * 4;      // [4][NP]
+    double* Upk = R0;                   // strict upper triangle of U by columns
+    // ---- assembly: coordinates, right-hand sides, tiles ----
+    S.X = R0;
+    S.SPH = S.X + 3 * n;
+    int tt = wt * WAVE + lane;
+    for (int j = tt; j < n; j += WAVE * W) {
+        const double* q = P.xyz + 3 * (size_t)P.st_idx[S.off + j];
+        S.X[3 * j] = q[0];
+        S.X[3 * j + 1] = q[1];
+        S.X[3 * j + 2] = q[2];
+        if (S.sph) {
+            double lam, th;
+            g_sph(q[0], q[1], q[2], lam, th);
+            S.SPH[2 * j] = lam;
+            S.SPH[2 * j + 1] = th;
+        }
+    }
+    if (tt == 0) flag[0] = 0;
+    for (int r = tt; r < MP; r += WAVE * W) pos[r] = (r < m) ? m : -1;  // -1: padding row, never a pivot
+    team_sync<W>();
+    for (int q = tt; q < m * T; q += WAVE * W) BS[q] = rhs_entry(S, q / T, q - (q / T) * T);
+    team_sync<W>();
+    d4v acc[RT][NCT];
+    double amax = 0.0;
+#pragma unroll
+    for (int rt = 0; rt < RT; rt++) {
+        const int r = 16 * rt + rl;
+#pragma unroll
+        for (int sl = 0; sl < NCT; sl++) {
+            const int c0 = 16 * (wt + W * sl) + cg;
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int c = c0 + 4 * v;
+                double val = 0.0;
+                if (r < m) {
+                    if (c < m) {
+                        val = m_entry(r, c, n, S.e2, S.X, P.sph);
+                        amax = fmax(amax, fabs(val));
+                    } else if (c < m + T) {
+                        val = BS[r * T + (c - m)];
+                    }
+                }
+                acc[rt][sl][v] = val;
+                __builtin_amdgcn_sched_barrier(0);  // one entry at a time: short live ranges
+            }
+        }
+    }
+    amax = wave_max(amax);
+    if (lane == 0) amaxw[wt] = amax;
+    team_sync<W>();  // the coordinates are read: R0 now holds the panel buffers
+    // ---- blocked elimination ----
+    for (int kp = 0; kp < m; kp += 4) {
+        lane = opaque_v(lane);
+        rl = lane & 15;
+        cg = lane >> 4;
+        const int npn = min(4, m - kp);
+        const int ctp = kp >> 4, owner = ctp % W, slot = ctp / W, gp = (kp & 15) >> 2;
+        double* Lb = Lbuf + ((kp >> 2) & 1) * (MP * 4);
+        if (wt == owner) {
+            // panel column kp + j sits in register gp of lane group j
+#pragma unroll
+            for (int sl = 0; sl < NCT; sl++)
+                if (sl == slot) {
+#pragma unroll
+                    for (int v = 0; v < 4; v++)
+                        if (v == gp) {
+#pragma unroll
+                            for (int rt = 0; rt < RT; rt++) Pbuf[(16 * rt + rl) * 4 + cg] = acc[rt][sl][v];
+                        }
+                }
+            wave_lds_sync();
+            double a[RR][4];
+            bool live[RR], was[RR];
+            int newq[RR];
+#pragma unroll
+            for (int h = 0; h < RR; h++) {
+                const int r = lane + WAVE * h;
+                if (r < MP) {
+                    const double2* d = (const double2*)(Pbuf + r * 4);
+                    const double2 x0 = d[0], x1 = d[1];
+                    a[h][0] = x0.x;
+                    a[h][1] = x0.y;
+                    a[h][2] = x1.x;
+                    a[h][3] = x1.y;
+                    live[h] = pos[r] == m;
+                } else {
+                    a[h][0] = a[h][1] = a[h][2] = a[h][3] = 0.0;
+                    live[h] = false;
+                }
+                was[h] = live[h];
+                newq[h] = 4;
+            }
+            bool bad = false;
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                if (q >= npn) break;  // uniform
+                unsigned long long key = 0;
+#pragma unroll
+                for (int h = 0; h < RR; h++)
+                    if (live[h]) {
+                        const unsigned long long kh = pivot_key(a[h][q], lane + WAVE * h);
+                        key = kh > key ? kh : key;
+                    }
+                key = wave_umax_uniform(key);
+                const int p = 255 - (int)(key & 0xffull);
+                const int pl = p & (WAVE - 1), ph = p / WAVE;
+                const double piv = readlane_f64((RR == 1 || ph == 0) ? a[0][q] : a[RR - 1][q], pl);
+                bad |= (key == 0) | (piv == 0.0);
+#pragma unroll
+                for (int h = 0; h < RR; h++)
+                    if (lane + WAVE * h == p) {
+                        live[h] = false;
+                        newq[h] = q;
+                        pos[p] = kp + q;
+                    }
+                if (lane == 0) {
+                    perm[kp + q] = p;
+                    dg[kp + q] = piv;
+                }
+                const double inv = 1.0 / piv;
+                // the panel's later columns; in a last, partial panel the columns past m are
+                // right-hand sides, which the matrix-core update below eliminates
+                double pr[4];
+#pragma unroll
+                for (int j = q + 1; j < 4; j++)
+                    pr[j] = (j < npn) ? readlane_f64((RR == 1 || ph == 0) ? a[0][j] : a[RR - 1][j], pl) : 0.0;
+#pragma unroll
+                for (int h = 0; h < RR; h++)
+                    if (live[h]) {
+                        const double l = a[h][q] * inv;
+                        a[h][q] = l;
+#pragma unroll
+                        for (int j = q + 1; j < 4; j++)
+                            if (j < npn) a[h][j] = a[h][j] - l * pr[j];
+                    }
+            }
+#pragma unroll
+            for (int h = 0; h < RR; h++) {
+                const int r = lane + WAVE * h;
+                if (r < MP) {
+                    double2* d = (double2*)(Pbuf + r * 4);
+                    d[0] = make_double2(a[h][0], a[h][1]);
+                    d[1] = make_double2(a[h][2], a[h][3]);
+                    double lv[4];
+#pragma unroll
+                    for (int j = 0; j < 4; j++) lv[j] = (was[h] && j < npn && j < newq[h]) ? a[h][j] : 0.0;
+                    double2* e = (double2*)(Lb + r * 4);
+                    e[0] = make_double2(lv[0], lv[1]);
+                    e[1] = make_double2(lv[2], lv[3]);
+                }
+            }
+            if (lane == 0 && bad) flag[0] = 1;
+            wave_lds_sync();
+#pragma unroll
+            for (int sl = 0; sl < NCT; sl++)
+                if (sl == slot) {
+#pragma unroll
+                    for (int v = 0; v < 4; v++)
+                        if (v == gp) {
+#pragma unroll
+                            for (int rt = 0; rt < RT; rt++) acc[rt][sl][v] = Pbuf[(16 * rt + rl) * 4 + cg];
+                        }
+                }
+        }
+        team_sync<W>();  // multipliers, pivots and pos of this panel are visible
+        const int kdone = kp + npn;  // columns < kdone are finished
+        int pq[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) pq[q] = (q < npn) ? perm[kp + q] : 0;
+#pragma unroll
+        for (int sl = 0; sl < NCT; sl++) {
+            const int ct = wt + W * sl;
+            if (16 * ct + 16 <= kdone || 16 * ct >= m + T) continue;  // finished or empty tile (uniform)
+            // the pivot rows' values on this tile's columns
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                if (q >= npn) break;
+                const int p = pq[q];
+                if (rl == (p & 15)) {
+#pragma unroll
+                    for (int rt = 0; rt < RT; rt++)
+                        if (rt == (p >> 4)) {
+#pragma unroll
+                            for (int v = 0; v < 4; v++) U12[q * NP + 16 * ct + 4 * v + cg] = acc[rt][sl][v];
+                        }
+                }
+            }
+            wave_lds_sync();
+            if (lane < 16) {  // U rows: the unit-lower 4x4 block applied, column 16 ct + lane
+                const int c = 16 * ct + lane;
+                double u[4];
+#pragma unroll
+                for (int q = 0; q < 4; q++) u[q] = (q < npn) ? U12[q * NP + c] : 0.0;
+#pragma unroll
+                for (int q = 1; q < 4; q++)
+                    if (q < npn) {
+#pragma unroll
+                        for (int j = 0; j < q; j++) u[q] = u[q] - Lb[pq[q] * 4 + j] * u[j];
+                    }
+                const bool fin = c < kdone;
+#pragma unroll
+                for (int q = 0; q < 4; q++) U12[q * NP + c] = (fin || q >= npn) ? 0.0 : u[q];
+            }
+            wave_lds_sync();
+            const double aop = -U12[cg * NP + 16 * ct + rl];
+#pragma unroll
+            for (int rt = 0; rt < RT; rt++) {
+                const double bop = Lb[(16 * rt + rl) * 4 + cg];
+                acc[rt][sl] = __builtin_amdgcn_mfma_f64_16x16x4f64(aop, bop, acc[rt][sl], 0, 0, 0);
+            }
+        }
+    }
+    team_sync<W>();  // every tile is final; the panel buffers are free
+    if (flag[0] != 0) {  // an exact zero pivot: NaN weights, the stencil flagged singular
+        if (wt == 0 && writer) {
+            for (int q = lane; q < n * T; q += WAVE) {
+                const int j = q / T, t = q - j * T;
+                P.w[(size_t)t * P.nnz + S.off + j] = NAN;
+            }
+            if (lane == 0) {
+                P.fail_flags[s] = 1;
+                if (P.diag) {
+                    P.diag[4 * (size_t)s] = NAN;
+                    P.diag[4 * (size_t)s + 1] = 0;
+                    P.diag[4 * (size_t)s + 2] = 0;
+                    P.diag[4 * (size_t)s + 3] = RBFX_PATH_SINGULAR;
+                }
+            }
+        }
+        if (W == 1 || LU_WAVES == W) return;  // the only team of its barriers
+        writer = false;  // other teams share the workgroup barriers below: run on, write nothing
+    }
+    // U part (columns right of each row's pivot step) and eliminated right-hand sides by pivot order
+    lane = opaque_v(lane);
+    rl = lane & 15;
+    cg = lane >> 4;
+    double um = 0.0;
+#pragma unroll
+    for (int rt = 0; rt < RT; rt++) {
+        const int r = 16 * rt + rl;
+        if (r >= m) continue;
+        const int i = pos[r];
+#pragma unroll
+        for (int sl = 0; sl < NCT; sl++) {
+            const int c0 = 16 * (wt + W * sl) + cg;
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int c = c0 + 4 * v;
+                const double val = acc[rt][sl][v];
+                if (c < m) {
+                    if (c >= i) um = fmax(um, fabs(val));
+                    if (c > i) Upk[(size_t)c * (c - 1) / 2 + i] = val;
+                } else if (c < m + T) {
+                    BS[i * T + (c - m)] = val;
+                }
+            }
+        }
+    }
+    um = wave_max(um);
+    if (lane == 0) umaxw[wt] = um;
+    team_sync<W>();
+    // back substitution, right-hand sides t = t0 + wt + W q (chunks of QC per wave); lane i
+    // holds unknown i (pivot order)
+    constexpr int QC = 4;
+    lane = opaque_v(lane);
+    tt = wt * WAVE + lane;
+    for (int t0 = 0; t0 < T; t0 += W * QC) {
+        double b[QC][RR];
+#pragma unroll
+        for (int h = 0; h < RR; h++) {
+            const int i = lane + WAVE * h;
+#pragma unroll
+            for (int q = 0; q < QC; q++) {
+                const int t = t0 + wt + W * q;
+                b[q][h] = (t < T && i < m) ? BS[i * T + t] : 0.0;
+            }
+        }
+        for (int k = m - 1; k >= 0; k--) {
+            const double* uc = Upk + (size_t)k * (k - 1) / 2;
+            const double ukk = dg[k];
+            double ucol[RR];
+#pragma unroll
+            for (int h = 0; h < RR; h++) ucol[h] = (lane + WAVE * h < k) ? uc[lane + WAVE * h] : 0.0;
+#pragma unroll
+            for (int q = 0; q < QC; q++) {
+                if (t0 + wt + W * q >= T) continue;
+                const double bk = readlane_f64((RR == 1 || k < WAVE) ? b[q][0] : b[q][RR - 1], k & (WAVE - 1));
+                const double x = bk / ukk;
+#pragma unroll
+                for (int h = 0; h < RR; h++) {
+                    const int i = lane + WAVE * h;
+                    if (i < k)
+                        b[q][h] -= ucol[h] * x;
+                    else if (i == k)
+                        b[q][h] = x;
+                }
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int q = 0; q < QC; q++) {
+            const int t = t0 + wt + W * q;
+            if (t >= T) continue;
+            const double scale = (P.types[t] == RBFX_HV) ? P.hv_scale : 1.0;
+#pragma unroll
+            for (int h = 0; h < RR; h++) {
+                const int i = lane + WAVE * h;
+                if (i < n && writer) P.w[(size_t)t * P.nnz + S.off + i] = b[q][h] * scale;
+                // x by unknown for the diagnostics (this wave's columns of BS: every unknown of
+                // these right-hand sides has been read above)
+                if (P.diag && i < m) BS[i * T + t] = b[q][h];
+            }
+        }
+    }
+    if (tt == 0 && writer) P.fail_flags[s] = 0;
+    if (!P.diag) return;  // uniform
+    // diagnostics: x by unknown is in BS; the coordinates are restaged over the packed U
+    team_sync<W>();  // back substitution has read the packed U
+    for (int j = tt; j < n; j += WAVE * W) {
+        const double* q = P.xyz + 3 * (size_t)P.st_idx[S.off + j];
+        S.X[3 * j] = q[0];
+        S.X[3 * j + 1] = q[1];
+        S.X[3 * j + 2] = q[2];
+        if (S.sph) {
+            double lam, th;
+            g_sph(q[0], q[1], q[2], lam, th);
+            S.SPH[2 * j] = lam;
+            S.SPH[2 * j + 1] = th;
+        }
+    }
+    team_sync<W>();
+    if (wt != 0 || !writer) return;
+    double am = 0.0, umx = 0.0;
+    for (int w = 0; w < W; w++) {
+        am = fmax(am, amaxw[w]);
+        umx = fmax(umx, umaxw[w]);
+    }
+    const double be = residual<double, MAX_T>(S, (double*)nullptr, BS);
+    if (lane == 0) {
+        P.diag[4 * (size_t)s] = be;
+        P.diag[4 * (size_t)s + 1] = umx / am;
+        P.diag[4 * (size_t)s + 2] = 0;
+        P.diag[4 * (size_t)s + 3] = RBFX_PATH_F64;
+    }
+}
+
+// Work unit: {first position in P.order, stencils (<= teams of the layout), layout | n << 8, m}.
+template <bool WIDE>
+__global__ void __launch_bounds__(LU_BLOCK) __attribute__((amdgpu_waves_per_eu(2))) weights_lu_kernel(WParams P, const int4* __restrict__ units, int n_units,
+                                                             int* __restrict__ counter) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ int s_unit;
+    for (;;) {
+        if (threadIdx.x == 0) s_unit = atomicAdd(counter, 1);
+        __syncthreads();
+        const int u = s_unit;
+        if (u >= n_units) return;  // the same value in every thread
+        const int4 U = units[u];
+        const int lay = U.z & 0xff, n = U.z >> 8, m = U.w;
+        switch (lay) {
+            case 0: lu_team<1, 2, 2>(P, U.x, U.y, n, m, smem); break;
+            case 1: lu_team<2, 3, 2>(P, U.x, U.y, n, m, smem); break;
+            case 2: lu_team<2, 4, 2>(P, U.x, U.y, n, m, smem); break;
+            case 3: lu_team<4, 6, 2>(P, U.x, U.y, n, m, smem); break;
+            case 4: lu_team<4, 8, 2>(P, U.x, U.y, n, m, smem); break;
+            default:
+                if constexpr (WIDE) lu_team<4, 8, 3>(P, U.x, U.y, n, m, smem);
+                break;
+        }
+        __syncthreads();  // the team LDS and s_unit are free for the next unit
+    }
+}
+
+__global__ void zero_int_kernel(int* p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
 }
 
 // bad[0] = 1 and bad[1] = the index when some stencil index is outside [0, n_nodes).
@@ -1052,28 +1568,49 @@ __global__ void mark_fallback_kernel(const int32_t* order, int count, double* di
     if (i < count && diag) diag[4 * (size_t)order[i] + 3] = RBFX_PATH_F32_FALLBACK_F64;
 }
 
-}  // namespace
 
-extern "C" int rbfx_weights(rbfx_ctx* ctx, int32_t n_stencils, const int32_t* d_st_off, const int32_t* d_st_idx,
-                            int32_t n_nodes, const double* d_xyz, const double* d_eps, uint32_t dertypes, int32_t dim,
-                            int32_t poly_deg, int32_t hv_k, double hv_scale, int32_t mode, double* d_w, double* d_diag,
-                            int32_t* n_singular) {
-    if (!ctx) return RBFX_E_INVALID;
-    if (n_singular) *n_singular = 0;
-    if (n_stencils <= 0 || !d_st_off || !d_st_idx || n_nodes <= 0 || !d_xyz || !d_eps || !d_w)
-        return rbfx_fail(ctx, RBFX_E_INVALID, "weights: bad arguments");
-    const uint32_t supported = RBFX_X | RBFX_Y | RBFX_Z | RBFX_LAPL | RBFX_R | RBFX_HV | RBFX_LAMBDA | RBFX_THETA |
-                               RBFX_LSFC | RBFX_XSFC | RBFX_YSFC | RBFX_ZSFC | RBFX_INTERP;
-    if (!dertypes || (dertypes & ~supported))
-        return rbfx_fail(ctx, RBFX_E_INVALID, "weights: unsupported derivative mask 0x%x", dertypes);
-    if (dim < 1 || dim > 3) return rbfx_fail(ctx, RBFX_E_INVALID, "weights: dim %d not in 1..3", dim);
-    if (poly_deg < -1 || (poly_deg >= 0 && ((poly_deg & 15) > 3 || (poly_deg & ~(15 | RBFX_POLY_SPHERE)))))
-        return rbfx_fail(ctx, RBFX_E_INVALID, "weights: poly_deg %d not in -1..3 (| RBFX_POLY_SPHERE)", poly_deg);
-    if (hv_k < 0 || hv_k > 16) return rbfx_fail(ctx, RBFX_E_INVALID, "weights: hv_k %d", hv_k);
-    if (mode != RBFX_SOLVE_F64_LU && mode != RBFX_SOLVE_F32_LU_F64_IR) return rbfx_fail(ctx, RBFX_E_INVALID, "weights: mode %d", mode);
-    RBFX_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+// Which FP64 solver runs: the persistent matrix-core LU (default), or, with RBFX_WEIGHTS_F64
+// set to "reg" or "lds" (read at each call), the per-class register-resident or one-wave LDS
+// kernels. All three give the same weights bit for bit.
+enum class F64Solver { LU, REG, LDS };
+F64Solver f64_solver() {
+    const char* e = std::getenv("RBFX_WEIGHTS_F64");
+    if (e && std::strcmp(e, "lds") == 0) return F64Solver::LDS;
+    if (e && std::strcmp(e, "reg") == 0) return F64Solver::REG;
+    return F64Solver::LU;
+}
+
+// Device scratch of one call: [order: n][flags: n + 2][units: n int4][counter]. 256-byte
+// aligned pieces.
+struct WScratch {
+    int32_t* order;
+    int* flags;
+    int4* units;
+    int* counter;
+};
+size_t wscratch_bytes(int n) {
+    auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+    return al(sizeof(int32_t) * (size_t)n) + al(sizeof(int) * ((size_t)n + 2)) + al(sizeof(int4) * (size_t)n) + 256;
+}
+WScratch wscratch_carve(void* base, int n) {
+    auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+    unsigned char* p = (unsigned char*)base;
+    WScratch w;
+    w.order = (int32_t*)p;
+    p += al(sizeof(int32_t) * (size_t)n);
+    w.flags = (int*)p;
+    p += al(sizeof(int) * ((size_t)n + 2));
+    w.units = (int4*)p;
+    p += al(sizeof(int4) * (size_t)n);
+    w.counter = (int*)p;
+    return w;
+}
+
+// The whole weights computation on a caller-provided scratch of wscratch_bytes(n_stencils).
+int weights_impl(rbfx_ctx* ctx, int32_t n_stencils, const int32_t* d_st_off, const int32_t* d_st_idx, int32_t n_nodes,
+                 const double* d_xyz, const double* d_eps, uint32_t dertypes, int32_t dim, int32_t poly_deg, int32_t hv_k,
+                 double hv_scale, int32_t mode, double* d_w, double* d_diag, int32_t* n_singular, void* d_scratch) {
     hipStream_t st = ctx->stream;
-
     WParams P;
     std::memset(&P, 0, sizeof(P));
     int T = 0;
@@ -1121,64 +1658,90 @@ extern "C" int rbfx_weights(rbfx_ctx* ctx, int32_t n_stencils, const int32_t* d_
             order[fill[n]++] = s;
         }
     }
-    int32_t* d_order = nullptr;
-    int* d_flags = nullptr;
+    const WScratch W = wscratch_carve(d_scratch, n_stencils);
+    int32_t* d_order = W.order;
+    int* d_flags = W.flags;
     hipError_t e;
-    // d_flags: one flag per stencil, then [index check flag, an offending index]
-    if ((e = hipMalloc(&d_order, sizeof(int32_t) * n_stencils)) != hipSuccess ||
-        (e = hipMalloc(&d_flags, sizeof(int) * ((size_t)n_stencils + 2))) != hipSuccess) {
-        hipFree(d_order);
-        return rbfx_fail(ctx, RBFX_E_NOMEM, "weights: %s", hipGetErrorString(e));
-    }
-    int rc = RBFX_OK;
-    auto cleanup = [&]() {
-        hipStreamSynchronize(st);
-        hipFree(d_order);
-        hipFree(d_flags);
-    };
     {  // every stencil index must name a node: checked on the device before any solve reads it
         int* d_bad = d_flags + n_stencils;
         int bad[2] = {0, 0};
         const int blocks = (int)std::min<long long>(4096, ((long long)P.nnz + 255) / 256);
-        if ((e = hipMemsetAsync(d_bad, 0, 2 * sizeof(int), st)) == hipSuccess) {
-            hipLaunchKernelGGL(check_index_kernel, dim3(std::max(blocks, 1)), dim3(256), 0, st, d_st_idx, P.nnz, n_nodes,
-                               d_bad);
-            if ((e = hipGetLastError()) == hipSuccess &&
-                (e = hipMemcpyAsync(bad, d_bad, sizeof(bad), hipMemcpyDeviceToHost, st)) == hipSuccess)
-                e = hipStreamSynchronize(st);
-        }
-        if (e != hipSuccess) {
-            cleanup();
-            return rbfx_fail(ctx, RBFX_E_HIP, "weights: %s", hipGetErrorString(e));
-        }
-        if (bad[0]) {
-            cleanup();
-            return rbfx_fail(ctx, RBFX_E_INVALID, "weights: stencil index %d out of range", bad[1]);
-        }
+        hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(64), 0, st, d_bad, 2);
+        hipLaunchKernelGGL(check_index_kernel, dim3(std::max(blocks, 1)), dim3(256), 0, st, d_st_idx, P.nnz, n_nodes, d_bad);
+        if ((e = hipGetLastError()) == hipSuccess &&
+            (e = hipMemcpyAsync(bad, d_bad, sizeof(bad), hipMemcpyDeviceToHost, st)) == hipSuccess)
+            e = hipStreamSynchronize(st);
+        if (e != hipSuccess) return rbfx_fail(ctx, RBFX_E_HIP, "weights: %s", hipGetErrorString(e));
+        if (bad[0]) return rbfx_fail(ctx, RBFX_E_INVALID, "weights: stencil index %d out of range", bad[1]);
     }
-    if ((e = hipMemcpyAsync(d_order, order.data(), sizeof(int32_t) * n_stencils, hipMemcpyHostToDevice, st)) != hipSuccess) {
-        cleanup();
-        return rbfx_fail(ctx, RBFX_E_HIP, "weights: %s", hipGetErrorString(e));
-    }
+    RBFX_HIP_CHECK(ctx, hipMemcpyAsync(d_order, order.data(), sizeof(int32_t) * n_stencils, hipMemcpyHostToDevice, st));
     P.fail_flags = d_flags;
+    P.order = d_order;
     int lds_attr = 0;
     if (hipDeviceGetAttribute(&lds_attr, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device) != hipSuccess || lds_attr <= 0)
         lds_attr = 64 * 1024;
     const size_t lds_limit = (size_t)lds_attr;
+    const F64Solver solver = f64_solver();
 
-    auto launch_classes = [&](bool ir, const int32_t* ord, const std::vector<int>& sizes_count,
-                              const std::vector<int>& sizes_start) -> int {
+    // One persistent launch of the matrix-core LU over the classes sizes_count / sizes_start of
+    // the order array (largest systems first). The units go to the scratch's unit table.
+    std::vector<int4> units;
+    auto launch_persistent = [&](const std::vector<int>& sizes_count, const std::vector<int>& sizes_start) -> int {
+        units.clear();
+        size_t lds = 0;
+        bool wide = false;
+        for (int n = MAX_M; n >= 1; n--) {
+            const int c = sizes_count[n];
+            if (!c) continue;
+            const int m = n + num_monomials(effective_degree(poly_deg, n));
+            const int id = lu_pick(m, T);
+            if (id < 0) return rbfx_fail(ctx, RBFX_E_TOO_LARGE, "weights: system %d x %d with %d RHS has no layout", m, m, T);
+            const LuLayout L = lu_layout(id);
+            wide |= id >= 5;
+            const int teams = LU_WAVES / L.W;
+            lds = std::max(lds, (size_t)teams * lu_team_bytes(L.W, L.RT, L.NCT, m, n, T));
+            for (int i = 0; i < c; i += teams)
+                units.push_back(make_int4(sizes_start[n] + i, std::min(teams, c - i), id | (n << 8), m));
+        }
+        if (units.empty()) return RBFX_OK;
+        if (lds + STATIC_LDS_RESERVE > lds_limit)
+            return rbfx_fail(ctx, RBFX_E_TOO_LARGE, "weights: matrix-core LU needs %zu B of LDS", lds);
+        hipError_t le;
+        if ((le = hipMemcpyAsync(W.units, units.data(), sizeof(int4) * units.size(), hipMemcpyHostToDevice, st)) != hipSuccess)
+            return rbfx_fail(ctx, RBFX_E_HIP, "weights: %s", hipGetErrorString(le));
+        hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(64), 0, st, W.counter, 1);
+        int per_cu = 0;
+        le = wide ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, weights_lu_kernel<true>, LU_BLOCK, lds)
+                  : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, weights_lu_kernel<false>, LU_BLOCK, lds);
+        if (le != hipSuccess || per_cu < 1)
+            return rbfx_fail(ctx, RBFX_E_HIP, "weights: no occupancy for the matrix-core LU (%zu B LDS)", lds);
+        const int grid = (int)std::min<long long>((long long)units.size(), (long long)per_cu * std::max(ctx->n_cu, 1));
+        WParams Q = P;
+        Q.fp32 = 0;
+        if (wide)
+            hipLaunchKernelGGL(weights_lu_kernel<true>, dim3(grid), dim3(LU_BLOCK), lds, st, Q, (const int4*)W.units,
+                               (int)units.size(), W.counter);
+        else
+            hipLaunchKernelGGL(weights_lu_kernel<false>, dim3(grid), dim3(LU_BLOCK), lds, st, Q, (const int4*)W.units,
+                               (int)units.size(), W.counter);
+        le = hipGetLastError();
+        if (le != hipSuccess) return rbfx_fail(ctx, RBFX_E_HIP, "weights: matrix-core LU launch failed: %s", hipGetErrorString(le));
+        return RBFX_OK;
+    };
+
+    // One launch per size class (the FP32 refinement kernel, and the FP64 register / LDS kernels).
+    auto launch_classes = [&](bool ir, const std::vector<int>& sizes_count, const std::vector<int>& sizes_start) -> int {
         for (int n = MAX_M; n >= 1; n--) {
             int c = sizes_count[n];
             if (!c) continue;
             WParams Q = P;
-            Q.order = ord + sizes_start[n];
+            Q.order = d_order + sizes_start[n];
             Q.count = c;
             Q.np = num_monomials(effective_degree(poly_deg, n));
             Q.m = n + Q.np;
             Q.fp32 = ir ? 1 : 0;
             const bool small_t = T <= 4;
-            if (!ir && !f64_use_lds()) {
+            if (!ir && solver == F64Solver::REG) {
                 const size_t rl = f64_reg_lds(Q.m, n, T);
                 if (rl + STATIC_LDS_RESERVE > lds_limit)
                     return rbfx_fail(ctx, RBFX_E_TOO_LARGE, "weights: system %d x %d with %d RHS needs %zu B of LDS", Q.m, Q.m, T, rl);
@@ -1201,17 +1764,22 @@ extern "C" int rbfx_weights(rbfx_ctx* ctx, int32_t n_stencils, const int32_t* d_
         }
         return RBFX_OK;
     };
+    auto launch_f64 = [&](const std::vector<int>& sizes_count, const std::vector<int>& sizes_start) -> int {
+        return solver == F64Solver::LU ? launch_persistent(sizes_count, sizes_start)
+                                       : launch_classes(false, sizes_count, sizes_start);
+    };
     std::vector<int> cstart(MAX_M + 1, 0);
     for (int n = 1; n <= MAX_M; n++) cstart[n] = start[n - 1] - cnt[n];
-    rc = launch_classes(mode == RBFX_SOLVE_F32_LU_F64_IR, d_order, cnt, cstart);
-    if (rc) { cleanup(); return rc; }
+    int rc = (mode == RBFX_SOLVE_F32_LU_F64_IR) ? launch_classes(true, cnt, cstart) : launch_f64(cnt, cstart);
+    if (rc) {
+        hipStreamSynchronize(st);
+        return rc;
+    }
 
     std::vector<int> flags(n_stencils);
     if ((e = hipMemcpyAsync(flags.data(), d_flags, sizeof(int) * n_stencils, hipMemcpyDeviceToHost, st)) != hipSuccess ||
-        (e = hipStreamSynchronize(st)) != hipSuccess) {
-        cleanup();
+        (e = hipStreamSynchronize(st)) != hipSuccess)
         return rbfx_fail(ctx, RBFX_E_HIP, "weights: %s", hipGetErrorString(e));
-    }
     if (mode == RBFX_SOLVE_F32_LU_F64_IR) {
         // FP64 re-solve of the stencils whose refinement did not converge
         std::vector<int32_t> redo;
@@ -1222,28 +1790,64 @@ extern "C" int rbfx_weights(rbfx_ctx* ctx, int32_t n_stencils, const int32_t* d_
         if (!redo.empty()) {
             int acc = 0;
             for (int n = MAX_M; n >= 1; n--) { rstart[n] = acc; acc += rcnt[n]; }
-            if ((e = hipMemcpyAsync(d_order, redo.data(), sizeof(int32_t) * redo.size(), hipMemcpyHostToDevice, st)) != hipSuccess) {
-                cleanup();
-                return rbfx_fail(ctx, RBFX_E_HIP, "weights: %s", hipGetErrorString(e));
+            RBFX_HIP_CHECK(ctx, hipMemcpyAsync(d_order, redo.data(), sizeof(int32_t) * redo.size(), hipMemcpyHostToDevice, st));
+            rc = launch_f64(rcnt, rstart);
+            if (rc) {
+                hipStreamSynchronize(st);
+                return rc;
             }
-            rc = launch_classes(false, d_order, rcnt, rstart);
-            if (rc) { cleanup(); return rc; }
-            if (d_diag) {
-                hipLaunchKernelGGL(mark_fallback_kernel, dim3((unsigned)((redo.size() + 255) / 256)), dim3(256), 0, st, d_order, (int)redo.size(), d_diag);
-            }
+            if (d_diag)
+                hipLaunchKernelGGL(mark_fallback_kernel, dim3((unsigned)((redo.size() + 255) / 256)), dim3(256), 0, st, d_order,
+                                   (int)redo.size(), d_diag);
             std::vector<int> f2(n_stencils);
             if ((e = hipMemcpyAsync(f2.data(), d_flags, sizeof(int) * n_stencils, hipMemcpyDeviceToHost, st)) != hipSuccess ||
-                (e = hipStreamSynchronize(st)) != hipSuccess) {
-                cleanup();
+                (e = hipStreamSynchronize(st)) != hipSuccess)
                 return rbfx_fail(ctx, RBFX_E_HIP, "weights: %s", hipGetErrorString(e));
-            }
             for (int32_t s : redo) flags[s] = f2[s];
         }
     }
     int sing = 0;
     for (int s = 0; s < n_stencils; s++) sing += (flags[s] == 1);
-    cleanup();
     if (n_singular) *n_singular = sing;
     if (sing) return rbfx_fail(ctx, RBFX_E_SINGULAR, "weights: %d stencil system(s) singular", sing);
     return RBFX_OK;
+}
+
+// Argument checks shared by the entry points; RBFX_OK when the call may proceed.
+int weights_check(rbfx_ctx* ctx, int32_t n_stencils, const int32_t* d_st_off, const int32_t* d_st_idx, int32_t n_nodes,
+                  const double* d_xyz, const double* d_eps, uint32_t dertypes, int32_t dim, int32_t poly_deg, int32_t hv_k,
+                  int32_t mode, double* d_w) {
+    if (n_stencils <= 0 || !d_st_off || !d_st_idx || n_nodes <= 0 || !d_xyz || !d_eps || !d_w)
+        return rbfx_fail(ctx, RBFX_E_INVALID, "weights: bad arguments");
+    const uint32_t supported = RBFX_X | RBFX_Y | RBFX_Z | RBFX_LAPL | RBFX_R | RBFX_HV | RBFX_LAMBDA | RBFX_THETA |
+                               RBFX_LSFC | RBFX_XSFC | RBFX_YSFC | RBFX_ZSFC | RBFX_INTERP;
+    if (!dertypes || (dertypes & ~supported))
+        return rbfx_fail(ctx, RBFX_E_INVALID, "weights: unsupported derivative mask 0x%x", dertypes);
+    if (dim < 1 || dim > 3) return rbfx_fail(ctx, RBFX_E_INVALID, "weights: dim %d not in 1..3", dim);
+    if (poly_deg < -1 || (poly_deg >= 0 && ((poly_deg & 15) > 3 || (poly_deg & ~(15 | RBFX_POLY_SPHERE)))))
+        return rbfx_fail(ctx, RBFX_E_INVALID, "weights: poly_deg %d not in -1..3 (| RBFX_POLY_SPHERE)", poly_deg);
+    if (hv_k < 0 || hv_k > 16) return rbfx_fail(ctx, RBFX_E_INVALID, "weights: hv_k %d", hv_k);
+    if (mode != RBFX_SOLVE_F64_LU && mode != RBFX_SOLVE_F32_LU_F64_IR) return rbfx_fail(ctx, RBFX_E_INVALID, "weights: mode %d", mode);
+    return RBFX_OK;
+}
+
+}  // namespace
+
+extern "C" int rbfx_weights(rbfx_ctx* ctx, int32_t n_stencils, const int32_t* d_st_off, const int32_t* d_st_idx,
+                            int32_t n_nodes, const double* d_xyz, const double* d_eps, uint32_t dertypes, int32_t dim,
+                            int32_t poly_deg, int32_t hv_k, double hv_scale, int32_t mode, double* d_w, double* d_diag,
+                            int32_t* n_singular) {
+    if (!ctx) return RBFX_E_INVALID;
+    if (n_singular) *n_singular = 0;
+    int rc = weights_check(ctx, n_stencils, d_st_off, d_st_idx, n_nodes, d_xyz, d_eps, dertypes, dim, poly_deg, hv_k, mode, d_w);
+    if (rc) return rc;
+    RBFX_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    void* d_scratch = nullptr;
+    hipError_t e = hipMalloc(&d_scratch, wscratch_bytes(n_stencils));
+    if (e != hipSuccess) return rbfx_fail(ctx, RBFX_E_NOMEM, "weights: %s", hipGetErrorString(e));
+    rc = weights_impl(ctx, n_stencils, d_st_off, d_st_idx, n_nodes, d_xyz, d_eps, dertypes, dim, poly_deg, hv_k, hv_scale, mode,
+                      d_w, d_diag, n_singular, d_scratch);
+    hipStreamSynchronize(ctx->stream);
+    hipFree(d_scratch);
+    return rc;
 }

@@ -154,11 +154,12 @@ def test_weights_bad_index_rejected(ctx):
 
 
 def test_weights_register_kernel_bit_identical(ctx, monkeypatch):
-    """The register-resident FP64 solve (the default) and the one-wave LDS solve give
+    """The three FP64 solvers - the persistent matrix-core blocked LU (the default), the
+    per-class register-resident elimination and the one-wave LDS elimination - give
     bit-identical weights and diagnostics (the same pivots and the same per-element operation
-    order) over every register layout (systems of 27..121 unknowns, 4 and 5 right-hand
-    sides), and both report an exactly singular large system."""
-    print("case: register vs LDS kernel", flush=True)
+    order) over every layout (systems of 27..121 unknowns, 4, 5 and 13 right-hand sides), and
+    all report an exactly singular large system."""
+    print("case: matrix-core LU vs register vs LDS kernel", flush=True)
     ALL_TYPES = (R.X | R.Y | R.Z | R.LAPL | R.R | R.HV | R.LAMBDA | R.THETA | R.LSFC | R.XSFC | R.YSFC | R.ZSFC |
                  R.INTERP)  # 13 right-hand sides: m + T up to 134
     N = 20000
@@ -174,8 +175,12 @@ def test_weights_register_kernel_bit_identical(ctx, monkeypatch):
     idx = np.concatenate([knn[k][r] for k, r in zip(sizes, rows)]).astype(np.int32)
     eps = np.array([c[k][0] * np.sqrt(N) - c[k][1] for k in sizes])
     got = {}
-    for kern in ("lds", "reg"):
-        monkeypatch.setenv("RBFX_WEIGHTS_F64", kern)
+    for kern in ("lds", "reg", "lu"):
+        if kern == "lu":
+            monkeypatch.delenv("RBFX_WEIGHTS_F64", raising=False)
+        else:
+            monkeypatch.setenv("RBFX_WEIGHTS_F64", kern)
+        print("  solver", kern, flush=True)
         for mask, deg in ((R.X | R.Y | R.Z | R.LAPL, 2), (R.X | R.Y | R.Z | R.LAPL | R.HV, 3), (ALL_TYPES, 3)):
             got[kern, mask] = ctx.weights(off, idx, xyz, eps, mask, dim=3, poly_deg=deg, hv_k=2, hv_scale=-1e-3,
                                           diag=True)
@@ -187,11 +192,53 @@ def test_weights_register_kernel_bit_identical(ctx, monkeypatch):
                         poly_deg=2)
         assert e.value.code == -5
     for kern, mask in got:
-        if kern != "reg":
+        if kern == "lds":
             continue
-        w, dg = got["reg", mask]
+        w, dg = got[kern, mask]
         assert np.isfinite(w).all()
-        assert np.array_equal(w, got["lds", mask][0]), mask
-        assert np.array_equal(dg, got["lds", mask][1]), mask
+        assert np.array_equal(w, got["lds", mask][0]), (kern, mask)
+        assert np.array_equal(dg, got["lds", mask][1]), (kern, mask)
         if mask != ALL_TYPES:  # the angular operators are singular at the poles' stencils
             assert dg[:, 0].max() <= 64 * 121 * U64
+
+
+def c3_mix(N, n_st, seed, ks=(17, 31, 50, 75, 101)):
+    """A sample of the uneven-stencil workload: n_st stencils of the N-node jittered sphere,
+    sizes drawn uniformly from ks, eps from the per-size table."""
+    xyz = R.nodes_fibonacci(N, 1e-3, 0)
+    rng = np.random.default_rng(seed)
+    sizes = rng.choice(ks, size=n_st)
+    rows = rng.choice(N, size=n_st, replace=False)
+    knn = R.knn_kdtree(xyz, max(ks))
+    c = {17: (0.026, 0.08), 31: (0.035, 0.1), 50: (0.044, 0.14), 75: (0.051, 0.15), 101: (0.058, 0.16)}
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int32)
+    idx = np.concatenate([knn[r, :k] for k, r in zip(sizes, rows)]).astype(np.int32)
+    eps = np.array([c[k][0] * np.sqrt(N) - c[k][1] for k in sizes])
+    return xyz, off, idx, eps, sizes
+
+
+def test_weights_timed_path_matches_diagnostic_and_oracle(ctx):
+    """The production call (no diagnostics: the path bench.py times) gives the same weights,
+    bit for bit, as the call with diagnostics, on the uneven 17..101 mix at degree 2 and 3;
+    those weights meet the backward-error bar and agree with the CPU restatement's weights
+    within the forward bound 64 kappa u of each system."""
+    print("case: timed path vs diagnostic path, uneven mix", flush=True)
+    xyz, off, idx, eps, sizes = c3_mix(200000, 3000, 5)
+    mask = R.X | R.Y | R.Z | R.LAPL
+    for deg in (2, 3):
+        w_t, none = ctx.weights(off, idx, xyz, eps, mask, dim=3, poly_deg=deg)
+        assert none is None
+        w_d, dg = ctx.weights(off, idx, xyz, eps, mask, dim=3, poly_deg=deg, diag=True)
+        assert np.array_equal(w_t, w_d), deg
+        mmax = 101 + (10 if deg == 2 else 20)
+        print(f"  deg {deg}: backward error max {dg[:, 0].max():.2e}", flush=True)
+        assert dg[:, 0].max() <= 64 * mmax * U64
+        w_ref, _, ns = O.weights(off, idx, xyz, eps, mask, dim=3, poly_deg=deg)
+        assert ns == 0
+        for s in range(0, 3000, 97):
+            a, b = off[s], off[s + 1]
+            M = O.stencil_system(xyz, idx[a:b], eps[s], mask, 3, deg, 0)[0]
+            kappa = np.linalg.cond(M)
+            ref = w_ref[:, a:b]
+            rel = np.abs(w_t[:, a:b] - ref).max() / np.abs(ref).max()
+            assert rel <= min(1.0, 64 * kappa * U64), (deg, s, rel, kappa)
