@@ -621,27 +621,31 @@ __host__ __device__ inline size_t reg_lds_bytes(int m, int n, int T, int wnj, in
 }
 
 template <int CTRL, int ROWMASK>
-__device__ __forceinline__ unsigned long long umax_dpp_step(unsigned long long v) {
-    const int lo = (int)(unsigned)v, hi = (int)(unsigned)(v >> 32);
-    const unsigned olo = (unsigned)__builtin_amdgcn_update_dpp(lo, lo, CTRL, ROWMASK, 0xf, false);
-    const unsigned ohi = (unsigned)__builtin_amdgcn_update_dpp(hi, hi, CTRL, ROWMASK, 0xf, false);
-    const unsigned long long o = ((unsigned long long)ohi << 32) | olo;
+__device__ __forceinline__ unsigned umax32_dpp_step(unsigned v) {
+    const unsigned o = (unsigned)__builtin_amdgcn_update_dpp((int)v, (int)v, CTRL, ROWMASK, 0xf, false);
     return o > v ? o : v;
 }
 
-// Maximum over the 64 lanes as a wave-uniform value (all lanes must be active): two quad
-// swaps and two row rotations give every lane its 16-lane row's maximum, two row broadcasts
-// fold rows 0..2 into row 3, and lane 63 is read.
+// Maximum of a 32-bit value over the 64 lanes as a wave-uniform value (all lanes must be
+// active): two quad swaps and two row rotations give every lane its 16-lane row's maximum, two
+// row broadcasts fold rows 0..2 into row 3, and lane 63 is read.
+__device__ __forceinline__ unsigned wave_umax32_uniform(unsigned v) {
+    v = umax32_dpp_step<0xB1, 0xf>(v);   // quad_perm [1,0,3,2]
+    v = umax32_dpp_step<0x4E, 0xf>(v);   // quad_perm [2,3,0,1]
+    v = umax32_dpp_step<0x124, 0xf>(v);  // row_ror:4
+    v = umax32_dpp_step<0x128, 0xf>(v);  // row_ror:8
+    v = umax32_dpp_step<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+    v = umax32_dpp_step<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
+    return (unsigned)__builtin_amdgcn_readlane((int)v, 63);
+}
+
+// Maximum of a 64-bit key over the 64 lanes, wave-uniform: the largest high word, then the
+// largest low word among the lanes holding it (the same order as a 64-bit compare).
 __device__ __forceinline__ unsigned long long wave_umax_uniform(unsigned long long v) {
-    v = umax_dpp_step<0xB1, 0xf>(v);   // quad_perm [1,0,3,2]
-    v = umax_dpp_step<0x4E, 0xf>(v);   // quad_perm [2,3,0,1]
-    v = umax_dpp_step<0x124, 0xf>(v);  // row_ror:4
-    v = umax_dpp_step<0x128, 0xf>(v);  // row_ror:8
-    v = umax_dpp_step<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
-    v = umax_dpp_step<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
-    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)v, 63);
-    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(v >> 32), 63);
-    return ((unsigned long long)hi << 32) | lo;
+    const unsigned hi = (unsigned)(v >> 32), lo = (unsigned)v;
+    const unsigned mh = wave_umax32_uniform(hi);
+    const unsigned ml = wave_umax32_uniform(hi == mh ? lo : 0u);
+    return ((unsigned long long)mh << 32) | ml;
 }
 
 __device__ __forceinline__ double readlane_f64(double v, int l) {
@@ -1046,29 +1050,29 @@ size_t f64_reg_lds(int m, int n, int T) {
 
 // ---- persistent blocked LU on the matrix cores (FP64 default) -----------------------------
 // One launch solves every stencil: LU_BLOCK-thread workgroups stay resident and take work
-// units (one large stencil, or one stencil per wave for small systems; largest systems first)
-// from a device counter. A stencil's [M | R] lives in registers as 16x16 tiles in the
-// accumulator layout of v_mfma_f64_16x16x4f64, transposed: register v of lane l of a tile
-// holds row l%16, column 4v + l/16. Wave w of a W-wave team owns the
-// column tiles w, w + W, ... (NCT of them); RT row tiles cover the m rows.
-// Partial-pivot LU in panels of 4 columns (the K of the instruction):
+// units (one large stencil, or one stencil per wave or wave pair for small systems; largest
+// systems first) from a device counter. A stencil's [M | R] lives in registers as 16x16 tiles
+// in the accumulator layout of v_mfma_f64_16x16x4f64: register v of lane l of a tile holds
+// row l%16, column 4v + l/16. Wave w of a W-wave team owns the column tiles w, w + W, ...
+// (NCT of them); RT row tiles cover the m rows.
+// Partial-pivot LU in panels of 8 columns (two K-steps of the instruction):
 //   1. the owner wave of the panel's column tile copies the panel to LDS and factors it
 //      row-per-lane: per column a pivot search over the unpivoted rows (the largest |a|,
 //      ties to the smaller row, as in the unblocked kernels), multipliers, and the update of
 //      the panel's remaining columns; the row order is kept in perm/pos instead of moving rows;
 //   2. it stores the multipliers as the B operand L[r][q] (zero for rows pivoted earlier; a
 //      row that became the pivot of column q keeps its multipliers of columns < q);
-//   3. every wave forms the panel's four U rows on its own columns (the pivot rows' values
-//      with the unit-lower 4x4 block applied), zero on finished columns, as the A operand;
-//   4. every wave updates its tiles with one v_mfma_f64_16x16x4f64 per (row tile, live
-//      column tile): D = C - U12^T L21^T, i.e. a[r][c] -= sum_q L[r][q] U[q][c].
+//   3. every wave forms the panel's eight U rows on its own columns (the pivot rows' values
+//      with the unit-lower 8x8 block applied), zero on finished columns, as the A operand;
+//   4. every wave updates its tiles with two v_mfma_f64_16x16x4f64 per (row tile, live
+//      column tile): a[r][c] -= sum_q L[r][q] U[q][c], q = 0..3 then 4..7.
 // The instruction computes D[i][j] = C[i][j] + sum_k A[i][k] B[k][j] with A[i][k] from lane
 // i + 16k, B[k][j] from lane j + 16k and D[i][j] in register i/4 of lane j + 16(i%4), and
 // accumulates the four products in order, each fused and rounded on its own (a probe of all
 // 256 outputs on random operands equals four chained fma). Here i is the column and j the
-// row: A[i][k] = -U[k][c], B[k][j] = L[r][k]. So every element
-// sees the operations of the unblocked register kernel in the same order and the weights are
-// bit-identical to it. The pivot rows' own updates in step 4 reproduce the U rows of step 3.
+// row: A[i][k] = -U[k][c], B[k][j] = L[r][k]. So every element sees the operations of the
+// unblocked register kernel in the same order and the weights are bit-identical to it. The
+// pivot rows' own updates in step 4 reproduce the U rows of step 3.
 // After the last panel each lane stores its rows' U part and eliminated right-hand sides to
 // LDS by pivot order, and back substitution runs as in the register kernel.
 typedef double d4v __attribute__((ext_vector_type(4)));
@@ -1103,10 +1107,12 @@ __host__ __device__ inline int lu_pick(int m, int T) {
 
 // LDS of one team, in doubles: [R0][BS: m T][dg: m][amax: 8][umax: 8] then ints
 // [perm: m][pos: 16 RT][flag: 4]. R0 holds the staged coordinates (5n) during assembly, the
-// panel, two multiplier buffers and the U-row buffer (12 * 16 RT + 4 * 16 W NCT) during the
-// factorisation, and the strict upper triangle by columns (m(m-1)/2) for back substitution.
+// panel, two multiplier buffers (each [8][16 RT + 4]) and the U-row buffer ([8][16 W NCT + 4])
+// during the factorisation, and the strict upper triangle by columns (m(m-1)/2) for back
+// substitution. The panel buffers are column-major with a padded stride, so that the lanes of
+// one column, and the four columns one wave reads at once, fall on different banks.
 __host__ __device__ inline size_t lu_r0(int W, int RT, int NCT, int m, int n) {
-    const size_t a = (size_t)5 * n, b = (size_t)12 * 16 * RT + (size_t)4 * 16 * W * NCT, c = (size_t)m * (m - 1) / 2;
+    const size_t a = (size_t)5 * n, b = (size_t)24 * (16 * RT + 4) + (size_t)8 * (16 * W * NCT + 4), c = (size_t)m * (m - 1) / 2;
     const size_t ab = a > b ? a : b;
     return ab > c ? ab : c;
 }
@@ -1132,13 +1138,17 @@ __device__ __forceinline__ void team_sync() {
         __syncthreads();
 }
 
+// Not inlined: each layout then gets its own register allocation (inlined into one kernel
+// the five layouts spill hundreds of VGPRs).
 // Solves the stencils order[start .. start + count) (count <= number of teams), each of n
 // nodes and m = n + np unknowns, one per team. One-wave teams without a stencil return at
 // once; larger teams without one repeat the first team's solve and write nothing, so that
 // every team passes the same workgroup barriers.
 template <int W, int RT, int NCT>
-__device__ __forceinline__ void lu_team(const WParams& P, int start, int count, int n, int m, unsigned char* smem) {
+__device__ __attribute__((noinline)) void lu_team(const WParams& P, int start, int count, int n, int m, unsigned char* smem) {
     constexpr int MP = 16 * RT, NP = 16 * W * NCT, RR = (MP + WAVE - 1) / WAVE;
+    constexpr int PS = MP + 4, US = NP + 4;  // strides of the panel / multiplier and U-row buffers
+    constexpr int KB = 8;                    // panel width
     const int wv = threadIdx.x / WAVE;
     int lane = opaque_v(threadIdx.x & (WAVE - 1));
     const int team = wv / W, wt = wv - team * W;
@@ -1171,9 +1181,9 @@ __device__ __forceinline__ void lu_team(const WParams& P, int start, int count, 
     int* perm = (int*)(umaxw + 8);
     int* pos = perm + m;
     int* flag = pos + MP;
-    double* Pbuf = R0;                  // [MP][4]
-    double* Lbuf = R0 + MP * 4;         // [2][MP][4]
-    double* U12 = R0 + 3 * MP * 4;      // [4][NP]
+    double* Pbuf = R0;                  // [KB][PS]
+    double* Lbuf = R0 + KB * PS;        // [2][KB][PS]
+    double* U12 = R0 + 3 * KB * PS;     // [KB][US]
     double* Upk = R0;                   // strict upper triangle of U by columns
     // ---- assembly: coordinates, right-hand sides, tiles ----
     S.X = R0;
@@ -1224,51 +1234,52 @@ __device__ __forceinline__ void lu_team(const WParams& P, int start, int count, 
     amax = wave_max(amax);
     if (lane == 0) amaxw[wt] = amax;
     team_sync<W>();  // the coordinates are read: R0 now holds the panel buffers
-    // ---- blocked elimination ----
-    for (int kp = 0; kp < m; kp += 4) {
+    // ---- blocked elimination, panels of KB = 8 columns (two K-steps of the instruction) ----
+    const int RTL = (m + 15) >> 4;  // row tiles holding rows < m (the others are padding)
+    for (int kp = 0; kp < m; kp += KB) {
         lane = opaque_v(lane);
         rl = lane & 15;
         cg = lane >> 4;
-        const int npn = min(4, m - kp);
+        const int npn = min(KB, m - kp);
         const int ctp = kp >> 4, owner = ctp % W, slot = ctp / W, gp = (kp & 15) >> 2;
-        double* Lb = Lbuf + ((kp >> 2) & 1) * (MP * 4);
+        double* Lb = Lbuf + ((kp / KB) & 1) * (KB * PS);
         if (wt == owner) {
-            // panel column kp + j sits in register gp of lane group j
+            // panel column kp + j sits in register gp + j/4 of lane group j%4
 #pragma unroll
             for (int sl = 0; sl < NCT; sl++)
                 if (sl == slot) {
 #pragma unroll
-                    for (int v = 0; v < 4; v++)
+                    for (int v = 0; v < 4; v += 2)
                         if (v == gp) {
 #pragma unroll
-                            for (int rt = 0; rt < RT; rt++) Pbuf[(16 * rt + rl) * 4 + cg] = acc[rt][sl][v];
+                            for (int rt = 0; rt < RT; rt++) {
+                                Pbuf[cg * PS + 16 * rt + rl] = acc[rt][sl][v];
+                                Pbuf[(4 + cg) * PS + 16 * rt + rl] = acc[rt][sl][v + 1];
+                            }
                         }
                 }
             wave_lds_sync();
-            double a[RR][4];
+            double a[RR][KB];
             bool live[RR], was[RR];
             int newq[RR];
 #pragma unroll
             for (int h = 0; h < RR; h++) {
                 const int r = lane + WAVE * h;
                 if (r < MP) {
-                    const double2* d = (const double2*)(Pbuf + r * 4);
-                    const double2 x0 = d[0], x1 = d[1];
-                    a[h][0] = x0.x;
-                    a[h][1] = x0.y;
-                    a[h][2] = x1.x;
-                    a[h][3] = x1.y;
+#pragma unroll
+                    for (int j = 0; j < KB; j++) a[h][j] = Pbuf[j * PS + r];
                     live[h] = pos[r] == m;
                 } else {
-                    a[h][0] = a[h][1] = a[h][2] = a[h][3] = 0.0;
+#pragma unroll
+                    for (int j = 0; j < KB; j++) a[h][j] = 0.0;
                     live[h] = false;
                 }
                 was[h] = live[h];
-                newq[h] = 4;
+                newq[h] = KB;
             }
             bool bad = false;
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
+            for (int q = 0; q < KB; q++) {
                 if (q >= npn) break;  // uniform
                 unsigned long long key = 0;
 #pragma unroll
@@ -1296,9 +1307,9 @@ __device__ __forceinline__ void lu_team(const WParams& P, int start, int count, 
                 const double inv = 1.0 / piv;
                 // the panel's later columns; in a last, partial panel the columns past m are
                 // right-hand sides, which the matrix-core update below eliminates
-                double pr[4];
+                double pr[KB];
 #pragma unroll
-                for (int j = q + 1; j < 4; j++)
+                for (int j = q + 1; j < KB; j++)
                     pr[j] = (j < npn) ? readlane_f64((RR == 1 || ph == 0) ? a[0][j] : a[RR - 1][j], pl) : 0.0;
 #pragma unroll
                 for (int h = 0; h < RR; h++)
@@ -1306,7 +1317,7 @@ __device__ __forceinline__ void lu_team(const WParams& P, int start, int count, 
                         const double l = a[h][q] * inv;
                         a[h][q] = l;
 #pragma unroll
-                        for (int j = q + 1; j < 4; j++)
+                        for (int j = q + 1; j < KB; j++)
                             if (j < npn) a[h][j] = a[h][j] - l * pr[j];
                     }
             }
@@ -1314,15 +1325,11 @@ __device__ __forceinline__ void lu_team(const WParams& P, int start, int count, 
             for (int h = 0; h < RR; h++) {
                 const int r = lane + WAVE * h;
                 if (r < MP) {
-                    double2* d = (double2*)(Pbuf + r * 4);
-                    d[0] = make_double2(a[h][0], a[h][1]);
-                    d[1] = make_double2(a[h][2], a[h][3]);
-                    double lv[4];
 #pragma unroll
-                    for (int j = 0; j < 4; j++) lv[j] = (was[h] && j < npn && j < newq[h]) ? a[h][j] : 0.0;
-                    double2* e = (double2*)(Lb + r * 4);
-                    e[0] = make_double2(lv[0], lv[1]);
-                    e[1] = make_double2(lv[2], lv[3]);
+                    for (int j = 0; j < KB; j++) {
+                        Pbuf[j * PS + r] = a[h][j];
+                        Lb[j * PS + r] = (was[h] && j < npn && j < newq[h]) ? a[h][j] : 0.0;
+                    }
                 }
             }
             if (lane == 0 && bad) flag[0] = 1;
@@ -1331,58 +1338,75 @@ __device__ __forceinline__ void lu_team(const WParams& P, int start, int count, 
             for (int sl = 0; sl < NCT; sl++)
                 if (sl == slot) {
 #pragma unroll
-                    for (int v = 0; v < 4; v++)
+                    for (int v = 0; v < 4; v += 2)
                         if (v == gp) {
 #pragma unroll
-                            for (int rt = 0; rt < RT; rt++) acc[rt][sl][v] = Pbuf[(16 * rt + rl) * 4 + cg];
+                            for (int rt = 0; rt < RT; rt++) {
+                                acc[rt][sl][v] = Pbuf[cg * PS + 16 * rt + rl];
+                                acc[rt][sl][v + 1] = Pbuf[(4 + cg) * PS + 16 * rt + rl];
+                            }
                         }
                 }
         }
         team_sync<W>();  // multipliers, pivots and pos of this panel are visible
         const int kdone = kp + npn;  // columns < kdone are finished
-        int pq[4];
+        // 1. the pivot rows' values on this wave's live tiles
 #pragma unroll
-        for (int q = 0; q < 4; q++) pq[q] = (q < npn) ? perm[kp + q] : 0;
+        for (int q = 0; q < KB; q++) {
+            if (q >= npn) break;
+            const int p = perm[kp + q];
+            if (rl == (p & 15)) {
 #pragma unroll
-        for (int sl = 0; sl < NCT; sl++) {
-            const int ct = wt + W * sl;
-            if (16 * ct + 16 <= kdone || 16 * ct >= m + T) continue;  // finished or empty tile (uniform)
-            // the pivot rows' values on this tile's columns
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-                if (q >= npn) break;
-                const int p = pq[q];
-                if (rl == (p & 15)) {
+                for (int sl = 0; sl < NCT; sl++) {
+                    const int ct = wt + W * sl;
+                    if (16 * ct + 16 <= kdone || 16 * ct >= m + T) continue;
 #pragma unroll
                     for (int rt = 0; rt < RT; rt++)
                         if (rt == (p >> 4)) {
 #pragma unroll
-                            for (int v = 0; v < 4; v++) U12[q * NP + 16 * ct + 4 * v + cg] = acc[rt][sl][v];
+                            for (int v = 0; v < 4; v++) U12[q * US + 16 * ct + 4 * v + cg] = acc[rt][sl][v];
                         }
                 }
             }
-            wave_lds_sync();
-            if (lane < 16) {  // U rows: the unit-lower 4x4 block applied, column 16 ct + lane
-                const int c = 16 * ct + lane;
-                double u[4];
+        }
+        wave_lds_sync();
+        // 2. U rows: the unit-lower KB x KB block applied; lanes 16 sl .. 16 sl + 15 take the
+        //    columns of tile slot sl
+        if (lane < 16 * NCT) {
+            const int ct = wt + W * (lane >> 4);
+            const int c = 16 * ct + (lane & 15);
+            if (16 * ct < m + T) {
+                double u[KB];
 #pragma unroll
-                for (int q = 0; q < 4; q++) u[q] = (q < npn) ? U12[q * NP + c] : 0.0;
+                for (int q = 0; q < KB; q++) u[q] = (q < npn) ? U12[q * US + c] : 0.0;
 #pragma unroll
-                for (int q = 1; q < 4; q++)
+                for (int q = 1; q < KB; q++)
                     if (q < npn) {
+                        const int pq = perm[kp + q];
 #pragma unroll
-                        for (int j = 0; j < q; j++) u[q] = u[q] - Lb[pq[q] * 4 + j] * u[j];
+                        for (int jj = 0; jj < q; jj++) u[q] = u[q] - Lb[jj * PS + pq] * u[jj];
                     }
                 const bool fin = c < kdone;
 #pragma unroll
-                for (int q = 0; q < 4; q++) U12[q * NP + c] = (fin || q >= npn) ? 0.0 : u[q];
+                for (int q = 0; q < KB; q++) U12[q * US + c] = (fin || q >= npn) ? 0.0 : u[q];
             }
-            wave_lds_sync();
-            const double aop = -U12[cg * NP + 16 * ct + rl];
+        }
+        wave_lds_sync();
+        // 3. trailing update, one instruction per K-step, row tile and live column tile
 #pragma unroll
-            for (int rt = 0; rt < RT; rt++) {
-                const double bop = Lb[(16 * rt + rl) * 4 + cg];
-                acc[rt][sl] = __builtin_amdgcn_mfma_f64_16x16x4f64(aop, bop, acc[rt][sl], 0, 0, 0);
+        for (int sl = 0; sl < NCT; sl++) {
+            const int ct = wt + W * sl;
+            if (16 * ct + 16 <= kdone || 16 * ct >= m + T) continue;  // finished or empty tile (uniform)
+#pragma unroll
+            for (int ks = 0; ks < KB / 4; ks++) {
+                if (4 * ks >= npn) break;  // the U rows of this K-step are all zero (uniform)
+                const double aop = -U12[(4 * ks + cg) * US + 16 * ct + rl];
+#pragma unroll
+                for (int rt = 0; rt < RT; rt++) {
+                    if (rt >= RTL) break;  // padding rows only (uniform)
+                    const double bop = Lb[(4 * ks + cg) * PS + 16 * rt + rl];
+                    acc[rt][sl] = __builtin_amdgcn_mfma_f64_16x16x4f64(aop, bop, acc[rt][sl], 0, 0, 0);
+                }
             }
         }
     }
@@ -1451,12 +1475,22 @@ __device__ __forceinline__ void lu_team(const WParams& P, int start, int count, 
                 b[q][h] = (t < T && i < m) ? BS[i * T + t] : 0.0;
             }
         }
+        // column k of U (rows < k) and the pivot u_kk are loaded one step ahead
+        double ucn[RR], ukn = dg[m - 1];
+#pragma unroll
+        for (int h = 0; h < RR; h++)
+            ucn[h] = (lane + WAVE * h < m - 1) ? Upk[(size_t)(m - 1) * (m - 2) / 2 + lane + WAVE * h] : 0.0;
         for (int k = m - 1; k >= 0; k--) {
-            const double* uc = Upk + (size_t)k * (k - 1) / 2;
-            const double ukk = dg[k];
             double ucol[RR];
 #pragma unroll
-            for (int h = 0; h < RR; h++) ucol[h] = (lane + WAVE * h < k) ? uc[lane + WAVE * h] : 0.0;
+            for (int h = 0; h < RR; h++) ucol[h] = ucn[h];
+            const double ukk = ukn;
+            if (k > 0) {
+                const double* uc = Upk + (size_t)(k - 1) * (k - 2) / 2;
+#pragma unroll
+                for (int h = 0; h < RR; h++) ucn[h] = (lane + WAVE * h < k - 1) ? uc[lane + WAVE * h] : 0.0;
+                ukn = dg[k - 1];
+            }
 #pragma unroll
             for (int q = 0; q < QC; q++) {
                 if (t0 + wt + W * q >= T) continue;
