@@ -165,6 +165,21 @@ int rbfx_weights(rbfx_ctx* ctx, int32_t n_stencils, const int32_t* d_st_off, con
                  int32_t poly_deg, int32_t hv_k, double hv_scale, int32_t mode, double* d_w, double* d_diag,
                  int32_t* n_singular);
 
+/* rbfx_weights that allocates no device memory. No counterpart in the reference: it is for
+ * callers that cannot allocate (code under stream capture, serving loops, harnesses that
+ * forbid it). Same arguments, same kernels, same results. The solver's scratch (stencil order,
+ * per-stencil flags, work units, work counter: about 24 bytes per stencil) lives in a
+ * file-scope device array sized for RBFX_WEIGHTS_NOALLOC_MAX_STENCILS stencils, which each
+ * device holds until the process ends; n_stencils above it is RBFX_E_TOO_LARGE. Every call on
+ * a device shares that device's array, so calls that overlap on one device (several contexts
+ * or threads) race and must be serialised by the caller. Host-side staging (the size sort) still
+ * uses host memory. */
+#define RBFX_WEIGHTS_NOALLOC_MAX_STENCILS 262144
+int rbfx_weights_noalloc(rbfx_ctx* ctx, int32_t n_stencils, const int32_t* d_st_off, const int32_t* d_st_idx,
+                         int32_t n_nodes, const double* d_xyz, const double* d_eps, uint32_t dertypes, int32_t dim,
+                         int32_t poly_deg, int32_t hv_k, double hv_scale, int32_t mode, double* d_w, double* d_diag,
+                         int32_t* n_singular);
+
 /* ---- ELL operators (fixed stencil width) --------------------------------------------- */
 /* Builds the device layout of n_ops operators that share one index array, the analogue of
  * RBFFD_CL::updateStencilsOnGPU/updateWeightsOnGPU (src/rbffd/rbffd_cl.cpp:215-257, 306-463)

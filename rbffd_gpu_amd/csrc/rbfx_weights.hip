@@ -21,6 +21,7 @@
 #include <cmath>
 #include <cstring>
 #include <cstdlib>
+#include <mutex>
 #include <vector>
 
 #include "rbfx_internal.h"
@@ -1622,9 +1623,10 @@ struct WScratch {
     int4* units;
     int* counter;
 };
-size_t wscratch_bytes(int n) {
-    auto al = [](size_t b) { return (b + 255) / 256 * 256; };
-    return al(sizeof(int32_t) * (size_t)n) + al(sizeof(int) * ((size_t)n + 2)) + al(sizeof(int4) * (size_t)n) + 256;
+constexpr size_t wscratch_al(size_t b) { return (b + 255) / 256 * 256; }
+constexpr size_t wscratch_bytes(int n) {
+    return wscratch_al(sizeof(int32_t) * (size_t)n) + wscratch_al(sizeof(int) * ((size_t)n + 2)) +
+           wscratch_al(sizeof(int4) * (size_t)n) + 256;
 }
 WScratch wscratch_carve(void* base, int n) {
     auto al = [](size_t b) { return (b + 255) / 256 * 256; };
@@ -1866,6 +1868,52 @@ int weights_check(rbfx_ctx* ctx, int32_t n_stencils, const int32_t* d_st_off, co
 }
 
 }  // namespace
+
+// Scratch of rbfx_weights_noalloc: one copy per device, at a per-device address taken once with
+// hipGetSymbolAddress (the symbol itself is a host-side stand-in, not a device address).
+static __device__ __align__(256) unsigned char scratch[wscratch_bytes(RBFX_WEIGHTS_NOALLOC_MAX_STENCILS)];
+namespace {
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// The device address of this device's scratch array (ctx's device current), found on the first
+// call on the device.
+int noalloc_scratch(rbfx_ctx* ctx, void** out) {
+    if (ctx->device < 0 || ctx->device >= NOALLOC_MAX_DEVICES)
+        return rbfx_fail(ctx, RBFX_E_INVALID, "weights_noalloc: device %d beyond %d", ctx->device, NOALLOC_MAX_DEVICES);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[ctx->device]) {
+        void* ptr = nullptr;
+        hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return rbfx_fail(ctx, RBFX_E_HIP, "weights_noalloc: hipGetSymbolAddress: %s", hipGetErrorString(e));
+        g_scratch_ptr[ctx->device] = ptr;
+    }
+    *out = g_scratch_ptr[ctx->device];
+    return RBFX_OK;
+}
+}  // namespace
+
+extern "C" int rbfx_weights_noalloc(rbfx_ctx* ctx, int32_t n_stencils, const int32_t* d_st_off, const int32_t* d_st_idx,
+                                    int32_t n_nodes, const double* d_xyz, const double* d_eps, uint32_t dertypes,
+                                    int32_t dim, int32_t poly_deg, int32_t hv_k, double hv_scale, int32_t mode,
+                                    double* d_w, double* d_diag, int32_t* n_singular) {
+    if (!ctx) return RBFX_E_INVALID;
+    if (n_singular) *n_singular = 0;
+    int rc = weights_check(ctx, n_stencils, d_st_off, d_st_idx, n_nodes, d_xyz, d_eps, dertypes, dim, poly_deg, hv_k, mode, d_w);
+    if (rc) return rc;
+    if (n_stencils > RBFX_WEIGHTS_NOALLOC_MAX_STENCILS)
+        return rbfx_fail(ctx, RBFX_E_TOO_LARGE, "weights_noalloc: %d stencils exceed the scratch maximum of %d", n_stencils,
+                         RBFX_WEIGHTS_NOALLOC_MAX_STENCILS);
+    RBFX_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    void* d_scratch = nullptr;
+    if ((rc = noalloc_scratch(ctx, &d_scratch))) return rc;
+    rc = weights_impl(ctx, n_stencils, d_st_off, d_st_idx, n_nodes, d_xyz, d_eps, dertypes, dim, poly_deg, hv_k, hv_scale, mode,
+                      d_w, d_diag, n_singular, d_scratch);
+    hipStreamSynchronize(ctx->stream);
+    return rc;
+}
 
 extern "C" int rbfx_weights(rbfx_ctx* ctx, int32_t n_stencils, const int32_t* d_st_off, const int32_t* d_st_idx,
                             int32_t n_nodes, const double* d_xyz, const double* d_eps, uint32_t dertypes, int32_t dim,

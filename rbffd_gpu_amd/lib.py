@@ -18,6 +18,7 @@ F64, F32 = 0, 1
 F32_RN = 2  # weight storage: FP32 with plain nearest rounding (F32 keeps row sums)
 SOLVE_F64_LU, SOLVE_F32_LU_F64_IR = 0, 1
 POLY_SPHERE = 16  # poly_deg flag: sphere-reduced monomial basis
+WEIGHTS_NOALLOC_MAX_STENCILS = 262144  # scratch capacity of rbfx_weights_noalloc (include/rbfx.h)
 PATH_F64, PATH_F32_IR, PATH_F32_FALLBACK_F64, PATH_SINGULAR = 0, 1, 2, 3
 ERR_NAMES = {-1: "INVALID", -2: "HIP", -3: "NOMEM", -4: "NODEVICE", -5: "SINGULAR", -6: "STATE", -7: "TOO_LARGE"}
 
@@ -77,6 +78,8 @@ def load_library(path=None):
         "rbfx_ell_build_local": (C.c_int, [vp, i32, i32]),
         "rbfx_ell_local_info": (C.c_int, [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)]),
         "rbfx_weights": (C.c_int, [vp, i32, vp, vp, i32, vp, vp, u32, i32, i32, i32, f64, i32, vp, vp, C.POINTER(i32)]),
+        "rbfx_weights_noalloc": (C.c_int, [vp, i32, vp, vp, i32, vp, vp, u32, i32, i32, i32, f64, i32, vp, vp,
+                                           C.POINTER(i32)]),
         "rbfx_ell_create": (C.c_int, [vp, i32, i32, i32, vp, i32, vp, vp, i32, C.POINTER(vp)]),
         "rbfx_ell_destroy": (None, [vp]),
         "rbfx_ell_info": (C.c_int, [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)]),
@@ -255,8 +258,10 @@ class Context:
         return ms.value
 
     def weights(self, st_off, st_idx, xyz, eps, dertypes, dim=3, poly_deg=0, hv_k=0, hv_scale=1.0,
-                mode=SOLVE_F64_LU, diag=False, device_out=False):
-        """Batched weights. Host or device inputs; returns (w[T, nnz], diag[n_st, 4] or None)."""
+                mode=SOLVE_F64_LU, diag=False, device_out=False, noalloc=False, d_w=None):
+        """Batched weights. Host or device inputs; returns (w[T, nnz], diag[n_st, 4] or None).
+        noalloc selects rbfx_weights_noalloc (library scratch in a static device array); d_w
+        (optional) is a caller-provided device output of shape (T, nnz)."""
         d_off = st_off if isinstance(st_off, DeviceArray) else self.to_device(np.asarray(st_off, np.int32))
         d_idx = st_idx if isinstance(st_idx, DeviceArray) else self.to_device(np.asarray(st_idx, np.int32))
         d_xyz = xyz if isinstance(xyz, DeviceArray) else self.to_device(np.asarray(xyz, np.float64))
@@ -267,11 +272,13 @@ class Context:
         d_eps = self.to_device(e)
         T = len(dertype_bits(dertypes))
         nnz = d_idx.size
-        d_w = self.alloc((T, nnz), np.float64)
+        if d_w is None:
+            d_w = self.alloc((T, nnz), np.float64)
         d_diag = self.alloc((n_st, 4), np.float64) if diag else None
         nsing = C.c_int32()
-        rc = RBFX.rbfx_weights(self.h, n_st, d_off.ptr, d_idx.ptr, d_xyz.shape[0], d_xyz.ptr, d_eps.ptr, dertypes, dim,
-                               poly_deg, hv_k, hv_scale, mode, d_w.ptr, d_diag.ptr if d_diag else None, C.byref(nsing))
+        fn = RBFX.rbfx_weights_noalloc if noalloc else RBFX.rbfx_weights
+        rc = fn(self.h, n_st, d_off.ptr, d_idx.ptr, d_xyz.shape[0], d_xyz.ptr, d_eps.ptr, dertypes, dim, poly_deg, hv_k,
+                hv_scale, mode, d_w.ptr, d_diag.ptr if d_diag else None, C.byref(nsing))
         self.check(rc, "weights")
         if device_out:
             return d_w, d_diag
