@@ -8,6 +8,10 @@
 //
 // usage: rbfx_cosine_bell <ico:L|fib:N> <n> <eps_c1> <eps_c2> <hv_k> <hv_gamma> <steps_per_rev>
 //                         <revolutions> <f64|f32> [alpha_denom=2] [dim=2] [numbering=morton]
+//                         [stencils=kdtree] [max_steps=0: all] [dump_weights=1 if N <= 100000]
+// max_steps stops the run after that many RK4 steps (for step-by-step comparisons);
+// dump_weights writes cosine_bell_weights.bin (HV, LAMBDA, THETA in stencil order).
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -25,7 +29,7 @@ int main(int argc, char** argv) {
     if (argc < 10) {
         std::fprintf(stderr,
                      "usage: %s ico:L|fib:N n c1 c2 hv_k hv_gamma steps_per_rev revs f64|f32 [alpha_denom] [dim] "
-                     "[morton|input] [kdtree|gpu]\n",
+                     "[morton|input] [kdtree|gpu] [max_steps] [dump_weights]\n",
                      argv[0]);
         return 2;
     }
@@ -56,6 +60,7 @@ int main(int argc, char** argv) {
     const bool morton = argc > 12 ? std::string(argv[12]) != "input" : true;
     const Grid::st_generator_t stencils =
         argc > 13 && std::string(argv[13]) == "gpu" ? Grid::ST_KDTREE_GPU : Grid::ST_KDTREE;
+    const long max_steps = argc > 14 ? std::atol(argv[14]) : 0;
     const double T = 1036800.0, a = 1.0;
     double alpha = alpha_denom != 0 ? M_PI / alpha_denom : 0.0;
 
@@ -80,7 +85,13 @@ int main(int argc, char** argv) {
     double dt = T / steps;
     rbfx_sync(der.getContext());
     auto t3 = std::chrono::steady_clock::now();
-    for (int r = 0; r < revs; r++) pde.advanceSteps(dt, steps);
+    long done = 0;
+    for (int r = 0; r < revs; r++) {
+        const long todo = max_steps > 0 ? std::min<long>(steps, max_steps - done) : steps;
+        if (todo <= 0) break;
+        pde.advanceSteps(dt, (int)todo);
+        done += todo;
+    }
     rbfx_sync(der.getContext());
     auto t4 = std::chrono::steady_clock::now();
     std::printf("============== Verifying Accuracy of Final Solution =============\n");
@@ -95,7 +106,8 @@ int main(int argc, char** argv) {
         std::fwrite(U.data(), 8, U.size(), o);
         std::fclose(o);
     }
-    if (N <= 100000) {  // weights of HV, LAMBDA, THETA in stencil order, for checking runs
+    const bool dump = argc > 15 ? std::atoi(argv[15]) != 0 : N <= 100000;
+    if (dump) {  // weights of HV, LAMBDA, THETA in stencil order, for checking runs
         FILE* w = std::fopen("cosine_bell_weights.bin", "wb");
         for (RBFFD::DerType t : {RBFFD::HV, RBFFD::LAMBDA, RBFFD::THETA})
             for (size_t s = 0; s < N; s++)
@@ -107,6 +119,6 @@ int main(int argc, char** argv) {
                 "\"linf\": %.17g, \"linfrel\": %.17g, \"max_backward_error\": %.3e, \"stencil_s\": %.3f, "
                 "\"weights_s\": %.3f, \"step_s\": %.4f, \"steps_per_s\": %.1f}\n",
                 N, n, der.getEpsilon()[0], hv_k, hv_gamma, dim, steps, revs, f32 ? "f32" : "f64", dt, nr[1], nr[2], nr[3],
-                nr[4], nr[5], be, sec(t0, t1), sec(t1, t2), sec(t3, t4), steps * revs / sec(t3, t4));
+                nr[4], nr[5], be, sec(t0, t1), sec(t1, t2), sec(t3, t4), done / sec(t3, t4));
     return 0;
 }
