@@ -109,6 +109,41 @@ def cpu_model():
     return None
 
 
+_ORACLE = None
+
+
+def cpu_oracle():
+    """The CPU restatement for the baseline legs, compiled -O3 -march=native (OpenMP,
+    -ffp-contract=off) for the host that runs the bench; the prebuilt x86-64-v3 build if the
+    host cannot compile. Returns (module, description of the build)."""
+    global _ORACLE
+    if _ORACLE is None:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import tempfile
+
+        import oracle as O
+
+        d = tempfile.mkdtemp(prefix="rbfx_oracle_")
+        path = O.build_native(d)
+        if path:
+            O.use_library(path)
+            _ORACLE = (O, "-O3 -march=native -fopenmp -ffp-contract=off")
+        else:
+            _ORACLE = (O, "-O3 -march=x86-64-v3 -fopenmp -ffp-contract=off (native build failed)")
+    return _ORACLE
+
+
+def median_rate(fn, runs=5, warmup=1):
+    """fn() -> (units, seconds); warm-up calls untimed, then the median rate of `runs` calls."""
+    for _ in range(warmup):
+        fn()
+    rates = []
+    for _ in range(runs):
+        units, sec = fn()
+        rates.append(units / sec)
+    return float(np.median(rates)), rates
+
+
 def stream_copy_gbs(ctx, R, nbytes=1 << 30, reps=10):
     """Measured HBM bandwidth of a plain copy kernel (read + write bytes / time)."""
     a = ctx.alloc(nbytes // 4, np.float32)
@@ -181,33 +216,40 @@ def bench_weights(ctx, R, N, cpu, nthreads, degrees=(2, 3), modes=("f64_lu", "f3
             outs[key].free()
         outs.clear()
     if cpu:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import oracle as O
-
+        O, build = cpu_oracle()
         rng = np.random.default_rng(2)
         sample = np.sort(rng.choice(N, size=100000, replace=False))
         s_off = np.concatenate([[0], np.cumsum(sizes[sample])]).astype(np.int32)
         s_idx = np.concatenate([idx[off[s]:off[s + 1]] for s in sample]).astype(np.int32)
-        t = time.perf_counter()
         w_cpu, dg_cpu, _ = O.weights(s_off, s_idx, xyz, eps[sample], mask, dim=3, poly_deg=2, diag=True,
                                      n_threads=nthreads)
-        tc = time.perf_counter() - t
+
+        def leg(ns, threads):
+            def f():
+                t = time.perf_counter()
+                O.weights(s_off[:ns + 1], s_idx[:s_off[ns]], xyz, eps[sample[:ns]], mask, dim=3, poly_deg=2,
+                          n_threads=threads)
+                return ns, time.perf_counter() - t
+            return f
+
+        rate, rates = median_rate(leg(len(sample), nthreads), runs=5, warmup=1)
         w_gpu, _ = ctx.weights(s_off, s_idx, xyz, eps[sample], mask, dim=3, poly_deg=2, diag=True)
         rel = np.abs(w_gpu - w_cpu).max() / np.abs(w_cpu).max()
-        res["cpu_baseline"] = {"stencils_per_s": round(len(sample) / tc, 1), "cores": nthreads, "kind": "port",
+        res["cpu_baseline"] = {"stencils_per_s": round(rate, 1), "cores": nthreads, "kind": "port",
                                "sample": f"{len(sample)} stencils of the same size mix, degree 2, FP64 LU "
-                                         f"(oracle/oracle.cpp), OpenMP over stencils, {tc:.1f} s",
+                                         f"(oracle/oracle.cpp, {build}), OpenMP over stencils; median of 5 timed "
+                                         f"runs after one warm-up run",
+                               "runs_stencils_per_s": [round(r, 1) for r in rates],
                                "gpu_vs_cpu_max_rel_weight_diff": float(rel),
                                "cpu_max_backward_error": float(dg_cpu[:, 0].max())}
         # the same solves on one host thread, on the first 10^4 stencils of the sample
         s1 = 10000
-        t = time.perf_counter()
-        O.weights(s_off[:s1 + 1], s_idx[:s_off[s1]], xyz, eps[sample[:s1]], mask, dim=3, poly_deg=2, n_threads=1)
-        t1 = time.perf_counter() - t
-        res["cpu_baseline"]["one_thread"] = {"stencils_per_s": round(s1 / t1, 1), "sample": f"{s1} stencils, {t1:.2f} s"}
+        r1, _ = median_rate(leg(s1, 1), runs=3, warmup=0)
+        res["cpu_baseline"]["one_thread"] = {"stencils_per_s": round(r1, 1),
+                                             "sample": f"{s1} stencils, median of 3 runs"}
         res["cpu_baseline"]["cpu_model"] = cpu_model()
         res["cpu_baseline"]["nproc"] = os.cpu_count()
-        res["gpu_over_cpu_deg2_f64"] = round(res["deg2_f64_lu"]["stencils_per_s"] / (len(sample) / tc), 1)
+        res["gpu_over_cpu_deg2_f64"] = round(res["deg2_f64_lu"]["stencils_per_s"] / rate, 1)
         log(f"weights CPU baseline: {res['cpu_baseline']}; GPU/CPU {res['gpu_over_cpu_deg2_f64']}x")
     return res
 
@@ -221,8 +263,8 @@ def main():
     ap.add_argument("--width", type=int, default=50)
     ap.add_argument("--fields", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-steps", type=int, default=0,
-                    help="CPU baseline RK4 steps (0: as many as fit in about 15 s after one probe step)")
+    ap.add_argument("--cpu-steps", type=int, default=1,
+                    help="CPU baseline RK4 steps per timed run (median of 5 runs after one warm-up run)")
     ap.add_argument("--kernel-only", action="store_true", help="set up, then launch only stage kernels (profiling)")
     ap.add_argument("--no-weights", action="store_true", help="skip the batched uneven-stencil weights section")
     ap.add_argument("--no-revolution", action="store_true", help="skip the untimed one-revolution accuracy run")
@@ -490,6 +532,28 @@ def main():
         b = N * n * (s_w + 4) + N * 2 * np.dtype(dt_np).itemsize
         spmv[name] = {"us": ms * 1e3, "GB_s": b / (ms * 1e-3) / 1e9, "bytes": b}
     log(f"SpMV: {spmv}")
+
+    # ---- fused SpMM (4 operators x F fields, standalone, every output written) ------------------
+    # algorithmic bytes: ops N n s_w + 4 N n (indices) + F N s_f (gathered fields, once) + ops F N s_out
+    spmm = {}
+    for name, ell, dt_np, ft, s_w in (("fp64", ell64, np.float64, R.F64, 8), ("fp32", ell32, np.float32, R.F32, 4)):
+        d_x = ctx.to_device(u0.astype(dt_np))
+        d_y = ctx.alloc((4, N, F), dt_np)
+        entry, _ = TU.tune_spmm(ctx, ell, F, d_x, d_y, ft, ft, table=table)
+        for _ in range(3):
+            ell.spmm(F, d_x, d_y, ft, ft)
+        ctx.timer_start()
+        for _ in range(R_L):
+            ell.spmm(F, d_x, d_y, ft, ft)
+        ms = ctx.timer_stop() / R_L
+        s_f = np.dtype(dt_np).itemsize
+        b = 4 * N * n * s_w + 4 * N * n + F * N * s_f + 4 * F * N * s_f
+        spmm[name] = {"us": round(ms * 1e3, 2), "GB_s": round(b / (ms * 1e-3) / 1e9, 1), "bytes": int(b),
+                      "frac_of_hbm_peak": round(b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                      "flop_per_launch": int(2 * 4 * F * N * n), "tuned": entry}
+        d_x.free()
+        d_y.free()
+    log(f"SpMM (4 ops x {F} fields): {spmm}")
     copy_gbs = stream_copy_gbs(ctx, R)
     log(f"plain copy kernel: {copy_gbs:.0f} GB/s")
 
@@ -504,26 +568,33 @@ def main():
     # ---- CPU baseline: the oracle's RK4 step on the same workload (all host cores) --------------
     cpu = None
     if not args.no_cpu:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import oracle as O
-
+        O, build = cpu_oracle()
         # the host's thread budget: OMP_NUM_THREADS when the environment sets it, else the affinity mask
         nthreads = int(os.environ.get("OMP_NUM_THREADS") or len(os.sched_getaffinity(0)))
         w_host = [v.download() for v in w_views]
         coefs_h = [None, -vel[:, 0].copy(), -vel[:, 1].copy(), -vel[:, 2].copy()]
-        if args.cpu_steps > 0:
-            _, cpu_steps, tc = O.rk4_step(st, w_host, coefs_h, u0, dt, n_steps=args.cpu_steps, n_threads=nthreads,
-                                          time_budget=1e30)
-        else:  # bounded sample: whole steps until about 15 s of wall time have been used
-            _, cpu_steps, tc = O.rk4_step(st, w_host, coefs_h, u0, dt, n_steps=30, n_threads=nthreads,
-                                          time_budget=15.0)
-        cpu = {"value": round(cpu_steps / tc, 4), "unit": "RK4 steps/s", "cores": nthreads, "kind": "port",
-               "sample": f"{cpu_steps} FP64 RK4 steps ({tc:.1f} s) of the same N={N}, n={n}, 4-operator, "
-                         f"{F}-field workload with the C++ restatement (oracle/oracle.cpp), OpenMP over rows",
+        steps_per_run = max(1, args.cpu_steps)
+
+        stp = O.RK4Stepper(st, w_host, coefs_h, F, n_threads=nthreads)  # weights stacked, workspace allocated once
+        u_cpu = np.array(u0, np.float64, order="C")
+
+        def leg(threads, k):
+            def f():
+                stp.n_threads = threads
+                t = time.perf_counter()
+                stp.step(u_cpu, dt, k)
+                return k, time.perf_counter() - t
+            return f
+
+        rate, rates = median_rate(leg(nthreads, steps_per_run), runs=5, warmup=1)
+        cpu = {"value": round(rate, 4), "unit": "RK4 steps/s", "cores": nthreads, "kind": "port",
+               "sample": f"FP64 RK4 steps of the same N={N}, n={n}, 4-operator, {F}-field workload with the C++ "
+                         f"restatement (oracle/oracle.cpp, {build}), OpenMP over rows, workspace allocated once; "
+                         f"median of 5 timed runs of {steps_per_run} step(s) after one warm-up run",
+               "runs_steps_per_s": [round(r, 4) for r in rates],
                "cpu_model": cpu_model(), "nproc": os.cpu_count()}
-        if args.cpu_steps <= 0:  # one step on one host thread, for the per-core figure
-            _, s1, t1 = O.rk4_step(st, w_host, coefs_h, u0, dt, n_steps=1, n_threads=1, time_budget=1e30)
-            cpu["one_thread"] = {"value": round(s1 / t1, 4), "sample": f"{s1} FP64 RK4 step, {t1:.1f} s"}
+        r1, _ = median_rate(leg(1, 1), runs=3, warmup=0)
+        cpu["one_thread"] = {"value": round(r1, 4), "sample": "1 FP64 RK4 step on one thread, median of 3 runs"}
         log(f"CPU baseline: {cpu}")
 
     value = K / wall32
@@ -565,6 +636,7 @@ def main():
             "stage_GB_s_fp64": round(ach64, 1),
             "copy_kernel_GB_s": round(copy_gbs, 1),
             "spmv": spmv,
+            "spmm_4ops": spmm,
             "weights_s": round(t_weights, 4),
             "weights_uneven": weights_c3,
             "knn": knn,
