@@ -9,6 +9,8 @@
 #include <cstring>
 #include <vector>
 
+#include <type_traits>
+
 #include "rbfx_ell_common.h"
 #include "rbfx_host_internal.h"
 
@@ -113,6 +115,46 @@ __global__ void __launch_bounds__(256) ell_spmm_kernel(const int32_t* __restrict
     }
 }
 
+// LDS-staged fused SpMM: one block per tile of the operator set's tile tables
+// (rbfx_ell_build_local). The tile's distinct neighbour records are copied into LDS once, then
+// every row gathers from there through 16-bit local indices; the summation order is that of
+// ell_spmm_kernel, so the outputs are the same bit for bit. MX: FP32 multiply-adds on the
+// matrix cores (16 fields on 4 lanes per row).
+template <typename WT, typename UT, typename AT, int NOPS, int FPT, int UB, int SG, int PD, bool MX>
+__global__ void __launch_bounds__(512) ell_spmm_lds_kernel(const int32_t* __restrict__ loc_pos,
+                                                           const int32_t* __restrict__ loc_off,
+                                                           const int32_t* __restrict__ loc_nodes,
+                                                           const uint16_t* __restrict__ loc_idx,
+                                                           const WT* __restrict__ w_t, const UT* __restrict__ u,
+                                                           UT* __restrict__ out, const int32_t* __restrict__ order,
+                                                           int n_rows, int width, int F, int n_tiles, int nb) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    UT* lds = reinterpret_cast<UT*>(smem);
+    const int tile = xcd_tile(blockIdx.x, nb);
+    if (tile >= n_tiles) return;  // uniform over the block
+    const int k0 = loc_off[tile];
+    ell_stage_tile<UT>(u, F, loc_nodes + k0, loc_off[tile + 1] - k0, lds);
+    __syncthreads();
+    const int t = threadIdx.x;
+    const int rl = t / SG;
+    const int p = loc_pos[tile] + rl;
+    if (p >= loc_pos[tile + 1]) return;  // after the only barrier
+    const int row = ell_row_at(order, p);
+    const int fofs = (t - rl * SG) * FPT;
+    AT d[NOPS][FPT];
+    if constexpr (MX)
+        ell_gather_dot_lds_mfma<NOPS, UB, PD>(loc_idx, (const float*)w_t, (const float*)lds, p, width, fofs, t - rl * SG, d);
+    else
+        ell_gather_dot_lds<WT, UT, AT, NOPS, FPT, UB, SG, PD>(loc_idx, w_t, lds, p, width, fofs, t - rl * SG, d);
+#pragma unroll
+    for (int o = 0; o < NOPS; o++) {
+        UT r[FPT];
+#pragma unroll
+        for (int f = 0; f < FPT; f++) r[f] = (UT)d[o][f];
+        store_pack<UT, FPT>(out + ((size_t)o * n_rows + row) * F + fofs, r);
+    }
+}
+
 // Single operator `op` out of an interleaved set of n_ops (one field).
 template <typename WT, typename UT, typename AT>
 __global__ void __launch_bounds__(256) ell_spmv_strided_kernel(const int32_t* __restrict__ idx_t,
@@ -187,8 +229,35 @@ __global__ void ell_reorder_kernel(const int32_t* __restrict__ idx_o, const WT* 
 
 inline int grid_blocks(int n_tiles) { return (n_tiles + 7) / 8 * 8; }
 
+// LDS-staged SpMM launch (share 4; share 5 with the matrix-core core): the block is the tile.
+template <typename WT, typename UT, typename AT, int NOPS>
+int launch_spmm_lds(rbfx_ell* e, const EllTune& t, int F, const void* u, void* out) {
+    if (e->lt <= 0) return rbfx_fail(e->ctx, RBFX_E_STATE, "spmm: LDS-staged shape without rbfx_ell_build_local");
+    const int fpt = t.fpt, ub = t.ub, G = F / fpt;
+    const int block = e->lt * G;
+    const size_t lds = (size_t)e->k_max * F * sizeof(UT);
+    const int n_tiles = e->n_ltiles, nb = grid_blocks(n_tiles);
+    hipStream_t s = e->ctx->stream;
+    const WT* wt = (const WT*)e->w_t;
+#define RBFX_SPMM_LDS(FPT_, UB_, SG_, MX_)                                                                              if (fpt == FPT_ && ub == UB_ && G == SG_) {                                                                              ell_spmm_lds_kernel<WT, UT, AT, NOPS, FPT_, UB_, SG_, 2, MX_><<<nb, block, lds, s>>>(                                    e->loc_pos, e->loc_off, e->loc_nodes, e->loc_idx, wt, (const UT*)u, (UT*)out, e->order, e->n_rows,                  e->width, F, n_tiles, nb);                                                                                       RBFX_LAUNCH_CHECK(e->ctx, "ell_spmm_lds_kernel");                                                                   return RBFX_OK;                                                                                                  }
+    if constexpr (NOPS >= 3) {
+        if (t.share == 5) {
+            if constexpr (std::is_same<WT, float>::value && std::is_same<UT, float>::value && std::is_same<AT, float>::value) {
+                RBFX_SPMM_LDS(4, 4, 4, true) RBFX_SPMM_LDS(4, 8, 4, true)
+            }
+        } else {
+            RBFX_SPMM_LDS(4, 8, 4, false) RBFX_SPMM_LDS(4, 4, 4, false) RBFX_SPMM_LDS(8, 8, 2, false)
+            RBFX_SPMM_LDS(8, 4, 2, false)
+        }
+    }
+#undef RBFX_SPMM_LDS
+    return rbfx_fail(e->ctx, RBFX_E_INVALID, "spmm: no LDS-staged kernel for fields/thread %d, batch %d, group %d, share %d",
+                     fpt, ub, G, t.share);
+}
+
 template <typename WT, typename UT, typename AT, int NOPS>
 int launch_spmm_fpt(rbfx_ell* e, const EllTune& t, int F, const void* u, void* out) {
+    if (t.share == 4 || t.share == 5) return launch_spmm_lds<WT, UT, AT, NOPS>(e, t, F, u, out);
     const int fpt = t.fpt, block = t.block;
     int G = F / fpt;
     int R = block / G;

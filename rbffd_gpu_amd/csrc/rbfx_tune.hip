@@ -43,7 +43,7 @@ bool lds_shape_fits(const rbfx_ell* e, int F, int ft, int acc, int kind, const E
     if (c.fpt <= 0 || F % c.fpt) return false;
     const int G = F / c.fpt;
     const size_t rec = (size_t)F * (ft == RBFX_F64 ? 8 : 4);
-    bool ok = kind == 1 && e->lt > 0 && e->n_ops >= 3 && e->lt * G <= 512 && rec % 16 == 0 &&
+    bool ok = (kind == 0 || kind == 1) && e->lt > 0 && e->n_ops >= 3 && e->lt * G <= 512 && rec % 16 == 0 &&
               (size_t)e->k_max * rec <= (size_t)e->lds_limit;
     if (c.share == 5) ok = ok && c.fpt == 4 && G == 4 && e->storage != RBFX_F64 && ft == RBFX_F32 && acc == RBFX_F32;
     return ok;
@@ -68,7 +68,7 @@ int rbfx_ell_pick(rbfx_ell* e, int F, int ft, int acc, int kind, EllTune* t) {
             // otherwise: the shape of a set without (fitting) tile tables, below
         }
     }
-    if (kind == 1 && e->lt > 0 && e->n_ops >= 3 && F == 16 && e->lt * 4 <= 512) {
+    if ((kind == 0 || kind == 1) && e->lt > 0 && e->n_ops >= 3 && F == 16 && e->lt * 4 <= 512) {
         const size_t rec = (size_t)F * (ft == RBFX_F64 ? 8 : 4);
         if ((size_t)e->k_max * rec <= (size_t)e->lds_limit) {
             const bool mx = e->storage != RBFX_F64 && ft == RBFX_F32 && acc == RBFX_F32;
@@ -97,7 +97,7 @@ int rbfx_tune_set(rbfx_ell* e, int32_t kind, int32_t n_fields, int32_t field_typ
     if (share == 4 || share == 5) {  // LDS-staged gathers (rbfx_ell_build_local); the block is the tile
         const int G = fpt > 0 ? n_fields / fpt : 0;
         const size_t rec = (size_t)n_fields * (field_type == RBFX_F64 ? 8 : 4);
-        bool ok = kind == 1 && e->lt > 0 && e->n_ops >= 3 && fpt > 0 && n_fields % fpt == 0 &&
+        bool ok = (kind == 0 || kind == 1) && e->lt > 0 && e->n_ops >= 3 && fpt > 0 && n_fields % fpt == 0 &&
                   ((fpt == 4 && G == 4) || (fpt == 8 && G == 2)) && (batch == 4 || batch == 8) &&
                   e->lt * G <= 512 && rec % 16 == 0 && (size_t)e->k_max * rec <= (size_t)e->lds_limit;
         // matrix-core multiply-adds: FP32 weights, fields and accumulation, 16 fields on 4 lanes

@@ -28,7 +28,7 @@ def candidates(n_fields, kind=KIND_SPMM, n_ops=1, local=None, field_bytes=8, fp3
     shapes (share 4; the block is the tile) when a tile's records fit in LDS_BYTES, and with
     FP32 fields and accumulation (fp32_accum) the staged shapes whose multiply-adds run on
     the matrix cores (share 5, 16 fields on 4 lanes)."""
-    if kind == KIND_RK4 and local and local[0] > 0 and n_ops >= 3 and (n_fields * field_bytes) % 16 == 0 \
+    if local and local[0] > 0 and n_ops >= 3 and (n_fields * field_bytes) % 16 == 0 \
             and local[1] * n_fields * field_bytes <= LDS_BYTES:
         for fpt, G in ((4, 4), (8, 2)):
             if n_fields == fpt * G and local[0] * G <= 512:
@@ -104,19 +104,21 @@ def tune_rk4(ctx, rk: RK4, ell: EllOperators, field_type, accum, dt=1e-6, reps=3
 
 def tune_spmm(ctx, ell: EllOperators, n_fields, d_u, d_out, field_type, accum, reps=5, table=None, log=None):
     results = []
-    for fpt, block, batch, share in candidates(n_fields, KIND_SPMM):
+    local = ell.local_info() if ell.n_ops >= 3 else None
+    fp32 = field_type == 1 and accum == 1 and ell.storage != 0
+    for fpt, block, batch, share in candidates(n_fields, KIND_SPMM, ell.n_ops, local, 8 if field_type == 0 else 4, fp32):
         ell.tune_set(KIND_SPMM, n_fields, field_type, accum, fpt, block, batch, share)
         ell.spmm(n_fields, d_u, d_out, field_type, accum)
         ctx.timer_start()
         for _ in range(reps):
             ell.spmm(n_fields, d_u, d_out, field_type, accum)
         ms = ctx.timer_stop() / reps
-        results.append((ms, fpt, block, batch))
+        results.append((ms, fpt, block, batch, share))
         if log:
-            log(f"  spmm shape fpt={fpt} block={block} batch={batch}: {ms * 1e3:.1f} us")
-    ms, fpt, block, batch = min(results)
-    ell.tune_set(KIND_SPMM, n_fields, field_type, accum, fpt, block, batch, 0)
-    entry = {"fpt": fpt, "block": block, "batch": batch, "us": round(ms * 1e3, 2)}
+            log(f"  spmm shape fpt={fpt} block={block} batch={batch} share={share}: {ms * 1e3:.1f} us")
+    ms, fpt, block, batch, share = min(results)
+    ell.tune_set(KIND_SPMM, n_fields, field_type, accum, fpt, block, batch, share)
+    entry = {"fpt": fpt, "block": block, "batch": batch, "share": share, "us": round(ms * 1e3, 2)}
     if table is not None:
         table[_key(ell, KIND_SPMM, n_fields, field_type, accum)] = entry
     return entry, sorted(results)

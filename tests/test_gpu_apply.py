@@ -139,6 +139,53 @@ def test_spmm_tuned_shapes_agree(ctx):
     ell.close()
 
 
+@pytest.mark.parametrize("storage", [R.F64, R.F32])
+def test_spmm_lds_staged_bit_identical(ctx, c1, storage):
+    """The LDS-staged SpMM shapes (tile records copied into LDS once, 16-bit local indices;
+    FP32: also the matrix-core multiply-adds) take the slots in the same order as the direct
+    gathers, so the fused 4-operator x 16-field outputs are bit-identical; FP64 also matches the
+    CPU restatement to 1e-12 (cancellation-normalised)."""
+    print(f"case: spmm LDS-staged shapes storage={storage}", flush=True)
+    from rbffd_gpu_amd import tune as TU
+    xyz, st = c1["xyz"], c1["st"]
+    off, idx = R.csr_from_fixed(st)
+    w, _, _ = O.weights(off, idx, xyz, 3.5, R.X | R.Y | R.Z | R.LSFC, dim=3, poly_deg=2)
+    ws = [w[t].reshape(st.shape) for t in range(4)]
+    F = 16
+    rng = np.random.default_rng(4)
+    u = rng.standard_normal((xyz.shape[0], F))
+    ft = R.F64 if storage == R.F64 else R.F32
+    dt = np.float64 if ft == R.F64 else np.float32
+    ell = R.EllOperators(ctx, st, ws, storage=storage)
+    ell.set_row_order(R.order_morton(xyz))
+    tl, k_max, n_tiles = ell.build_local(64, 128)
+    d_u = ctx.to_device(u.astype(dt))
+    outs = {}
+    for fpt, block, batch, share in TU.candidates(F, TU.KIND_SPMM, 4, (tl, k_max, n_tiles), 8 if ft == R.F64 else 4,
+                                                  ft == R.F32):
+        if share not in (4, 5) and block != 256:
+            continue
+        ell.tune_set(TU.KIND_SPMM, F, ft, ft, fpt, block, batch, share)
+        d_o = ctx.alloc((4, xyz.shape[0], F), dt)
+        ell.spmm(F, d_u, d_o, ft, ft)
+        outs[(fpt, batch, share)] = d_o.download()
+        d_o.free()
+    R.RBFX.rbfx_tune_clear()
+    staged = [k for k in outs if k[2] in (4, 5)]
+    assert staged and (ft == R.F64 or any(k[2] == 5 for k in staged))
+    base = outs[(4, 8, 0)]
+    for k, v in outs.items():
+        assert np.array_equal(v, base), k
+    if ft == R.F64:
+        for t in range(4):
+            for f in (0, 7, 15):
+                ref = O.apply(off, idx, ws[t].ravel(), u[:, f])
+                absum = (np.abs(ws[t]) * np.abs(u[st, f])).sum(1)
+                assert rel_apply_err(base[t, :, f], ref, absum) <= 1e-12
+    d_u.free()
+    ell.close()
+
+
 def test_ell_rejects_out_of_range_index(ctx):
     rng = np.random.default_rng(1)
     idx = rng.integers(0, 100, size=(100, 5)).astype(np.int32)
