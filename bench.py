@@ -161,6 +161,51 @@ def stream_copy_gbs(ctx, R, nbytes=1 << 30, reps=10):
 C3_EPS = {17: (0.026, 0.08), 31: (0.035, 0.1), 50: (0.044, 0.14), 75: (0.051, 0.15), 101: (0.058, 0.16)}
 
 
+KAPPA_EDGES = (6, 8, 10, 12, 14)  # log10 bin edges of the 2-norm condition number
+
+
+def kappa_bins(xyz, off, idx, eps, mask, deg, runs, n_sample=1500, seed=3):
+    """Weight error by condition number on a seeded sample of the stencils: each stencil's
+    system is assembled by the CPU restatement, kappa = cond_2(M) (numpy SVD), and its
+    reference weights are the restatement's FP64 LU. Per bin and solve mode: stencil count,
+    max normwise backward error (the GPU diagnostics), max forward difference
+    max|w - w_ref| / max|w_ref|, that difference over kappa * u, and the FP64 fallbacks of the
+    FP32 + refinement mode."""
+    O, _ = cpu_oracle()
+    u64 = np.finfo(np.float64).eps / 2
+    rng = np.random.default_rng(seed)
+    sample = np.sort(rng.choice(off.size - 1, size=n_sample, replace=False))
+    s_off = np.concatenate([[0], np.cumsum(np.diff(off)[sample])]).astype(np.int32)
+    s_idx = np.concatenate([idx[off[s]:off[s + 1]] for s in sample]).astype(np.int32)
+    w_ref, _, _ = O.weights(s_off, s_idx, xyz, eps[sample], mask, dim=3, poly_deg=deg)
+    kap = np.array([np.linalg.cond(O.stencil_system(xyz, idx[off[s]:off[s + 1]], eps[s], mask, 3, deg, 0)[0])
+                    for s in sample])
+    lk = np.log10(kap)
+    names = ["<1e6"] + [f"1e{a}-1e{b}" for a, b in zip(KAPPA_EDGES[:-1], KAPPA_EDGES[1:])] + [f">=1e{KAPPA_EDGES[-1]}"]
+    which = np.searchsorted(np.array(KAPPA_EDGES, float), lk, side="right")
+    out = {}
+    for b, name in enumerate(names):
+        sel = np.nonzero(which == b)[0]
+        if sel.size == 0:
+            continue
+        entry = {"stencils": int(sel.size), "kappa_max": float(kap[sel].max())}
+        for mode, (w, dg) in runs.items():
+            fwd, ratio = [], []
+            for q in sel:
+                s = sample[q]
+                ref = w_ref[:, s_off[q]:s_off[q + 1]]
+                d = np.abs(w[:, off[s]:off[s + 1]] - ref).max() / np.abs(ref).max()
+                fwd.append(d)
+                ratio.append(d / (kap[q] * u64))
+            e = {"max_backward_error": float(np.nanmax(dg[sample[sel], 0])), "max_forward_diff": float(max(fwd)),
+                 "max_forward_diff_over_kappa_u": float(max(ratio))}
+            if mode == "f32_lu_f64_ir":
+                e["fp64_fallbacks"] = int((dg[sample[sel], 3].astype(int) == 2).sum())
+            entry[mode] = e
+        out[name] = entry
+    return out
+
+
 def bench_weights(ctx, R, N, cpu, nthreads, degrees=(2, 3), modes=("f64_lu", "f32_lu_f64_ir"), eps_scale=1.0):
     """Batched weights with uneven stencils: N jittered sphere nodes, each stencil the n nearest
     with n drawn uniformly from {17, 31, 50, 75, 101} (seed 1), eps = c1 sqrt(N) - c2 per size
@@ -187,6 +232,7 @@ def bench_weights(ctx, R, N, cpu, nthreads, degrees=(2, 3), modes=("f64_lu", "f3
     outs = {}
     all_modes = {"f64_lu": R.SOLVE_F64_LU, "f32_lu_f64_ir": R.SOLVE_F32_LU_F64_IR}
     for deg in degrees:
+        dgs = {}
         for mode_name, mode in ((k, all_modes[k]) for k in modes):
             ctx.sync()
             t = time.perf_counter()
@@ -199,6 +245,7 @@ def bench_weights(ctx, R, N, cpu, nthreads, degrees=(2, 3), modes=("f64_lu", "f3
                                     device_out=True)
             dg = d_dg.download()
             d_dg.free()
+            dgs[mode_name] = dg
             paths = dg[:, 3].astype(int)
             entry = {"s": round(sec, 4), "stencils_per_s": round(N / sec, 1),
                      "max_backward_error": float(np.nanmax(dg[:, 0])),
@@ -212,6 +259,11 @@ def bench_weights(ctx, R, N, cpu, nthreads, degrees=(2, 3), modes=("f64_lu", "f3
             scale = np.abs(w64).max(axis=0)
             res[f"deg{deg}_ir_vs_f64_max_rel"] = float((np.abs(w32 - w64).max(axis=0) /
                                                         np.maximum(scale, 1e-300)).max())
+            if cpu:
+                res[f"deg{deg}_kappa_bins"] = kappa_bins(xyz, off, idx, eps, mask, deg,
+                                                         {"f64_lu": (w64, dgs["f64_lu"]),
+                                                          "f32_lu_f64_ir": (w32, dgs["f32_lu_f64_ir"])})
+                log(f"weights deg {deg} error by condition number: {res[f'deg{deg}_kappa_bins']}")
         for key in list(outs):
             outs[key].free()
         outs.clear()
