@@ -242,3 +242,31 @@ def test_weights_timed_path_matches_diagnostic_and_oracle(ctx):
             ref = w_ref[:, a:b]
             rel = np.abs(w_t[:, a:b] - ref).max() / np.abs(ref).max()
             assert rel <= min(1.0, 64 * kappa * U64), (deg, s, rel, kappa)
+
+
+def test_weights_f32_ir_uneven_mix(ctx):
+    """FP32 LU + FP64 refinement on the uneven 17..101 mix (degree 2 and 3): the stencils it
+    reports as converged meet the FP64 backward-error bar and agree with the oracle within
+    64 kappa u; the stencils that fell back to FP64 are bit-identical to the FP64 solve."""
+    print("case: fp32 + IR, uneven mix", flush=True)
+    xyz, off, idx, eps, sizes = c3_mix(200000, 3000, 6)
+    mask = R.X | R.Y | R.Z | R.LAPL
+    for deg in (2, 3):
+        w64, _ = ctx.weights(off, idx, xyz, eps, mask, dim=3, poly_deg=deg)
+        w32, dg = ctx.weights(off, idx, xyz, eps, mask, dim=3, poly_deg=deg, diag=True, mode=R.SOLVE_F32_LU_F64_IR)
+        paths = dg[:, 3].astype(int)
+        conv = np.nonzero(paths == R.lib.PATH_F32_IR)[0]
+        fb = np.nonzero(paths == R.lib.PATH_F32_FALLBACK_F64)[0]
+        print(f"  deg {deg}: converged {conv.size}, FP64 fallback {fb.size}", flush=True)
+        assert conv.size + fb.size == off.size - 1 and conv.size > 0 and fb.size > 0
+        mmax = 101 + (10 if deg == 2 else 20)
+        assert dg[conv, 0].max() <= 64 * mmax * U64
+        for s in fb:
+            assert np.array_equal(w32[:, off[s]:off[s + 1]], w64[:, off[s]:off[s + 1]]), s
+        w_ref, _, _ = O.weights(off, idx, xyz, eps, mask, dim=3, poly_deg=deg)
+        for s in conv[::max(1, conv.size // 40)]:
+            a, b = off[s], off[s + 1]
+            kappa = np.linalg.cond(O.stencil_system(xyz, idx[a:b], eps[s], mask, 3, deg, 0)[0])
+            ref = w_ref[:, a:b]
+            rel = np.abs(w32[:, a:b] - ref).max() / np.abs(ref).max()
+            assert rel <= min(1.0, 64 * kappa * U64), (deg, s, rel, kappa)
