@@ -206,7 +206,8 @@ def kappa_bins(xyz, off, idx, eps, mask, deg, runs, n_sample=1500, seed=3):
     return out
 
 
-def bench_weights(ctx, R, N, cpu, nthreads, degrees=(2, 3), modes=("f64_lu", "f32_lu_f64_ir"), eps_scale=1.0):
+def bench_weights(ctx, R, N, cpu, nthreads, degrees=(2, 3), modes=("f64_lu", "f32_lu_f64_ir"), eps_scale=1.0,
+                  size_set=(17, 31, 50, 75, 101)):
     """Batched weights with uneven stencils: N jittered sphere nodes, each stencil the n nearest
     with n drawn uniformly from {17, 31, 50, 75, 101} (seed 1), eps = c1 sqrt(N) - c2 per size
     (the reference's table, n = 75 interpolated), X/Y/Z/LAPL (dim 3), degree 2 and 3; FP64 LU and
@@ -215,11 +216,12 @@ def bench_weights(ctx, R, N, cpu, nthreads, degrees=(2, 3), modes=("f64_lu", "f3
     times a bounded sample of the same stencils on the host threads and checks the GPU weights
     of that sample (backward error, forward difference against the per-stencil condition).
     eps_scale multiplies every eps (1: the reference's table; larger: less flat, better
-    conditioned systems)."""
+    conditioned systems). size_set restricts the sizes drawn (one size: one class, for per-class
+    timing)."""
     t0 = time.time()
     xyz = R.nodes_fibonacci(N, 1e-3, 0)
     knn = R.knn_kdtree(xyz, 101)
-    sizes = np.random.default_rng(1).choice([17, 31, 50, 75, 101], size=N)
+    sizes = np.random.default_rng(1).choice(list(size_set), size=N)
     off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int32)
     mask_cols = np.arange(101)[None, :] < sizes[:, None]
     idx = knn[mask_cols].astype(np.int32)
@@ -228,7 +230,7 @@ def bench_weights(ctx, R, N, cpu, nthreads, degrees=(2, 3), modes=("f64_lu", "f3
     log(f"weights workload: {N} stencils, {int(off[-1])} nonzeros, setup {time.time() - t0:.1f}s")
     d_xyz, d_off, d_idx = ctx.to_device(xyz), ctx.to_device(off), ctx.to_device(idx)
     mask = R.X | R.Y | R.Z | R.LAPL
-    res = {"N": N, "sizes": [17, 31, 50, 75, 101], "operators": "X,Y,Z,LAPL (dim 3)", "eps_scale": eps_scale}
+    res = {"N": N, "sizes": [int(k) for k in size_set], "operators": "X,Y,Z,LAPL (dim 3)", "eps_scale": eps_scale}
     outs = {}
     all_modes = {"f64_lu": R.SOLVE_F64_LU, "f32_lu_f64_ir": R.SOLVE_F32_LU_F64_IR}
     for deg in degrees:
@@ -326,6 +328,8 @@ def main():
                     help="run only the batched uneven-stencil weights section (profiling)")
     ap.add_argument("--weights-f64-deg2", action="store_true",
                     help="with --weights-only: only the FP64 LU solve at degree 2 (one launch per size class)")
+    ap.add_argument("--weights-sizes", default="17,31,50,75,101",
+                    help="with --weights-only: comma-separated stencil sizes drawn uniformly (each of 17, 31, 50, 75, 101)")
     ap.add_argument("--node-order", action="store_true",
                     help="process rows in node order instead of the Morton locality order")
     ap.add_argument("--spiral-numbering", action="store_true",
@@ -343,7 +347,10 @@ def main():
     if args.weights_only:
         ctx = R.Context(0)
         sel = dict(degrees=(2,), modes=("f64_lu",)) if args.weights_f64_deg2 else {}
-        res = bench_weights(ctx, R, N, False, 1, eps_scale=args.weights_eps_scale, **sel)
+        size_set = tuple(int(k) for k in args.weights_sizes.split(","))
+        if not size_set or any(k not in C3_EPS for k in size_set):
+            sys.exit(f"--weights-sizes: each size must be one of {sorted(C3_EPS)}")
+        res = bench_weights(ctx, R, N, False, 1, eps_scale=args.weights_eps_scale, size_set=size_set, **sel)
         log("weights-only done", json.dumps(res))
         return
     t0 = time.time()

@@ -22,6 +22,7 @@
 #include <cstring>
 #include <cstdlib>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 #include "rbfx_internal.h"
@@ -391,7 +392,8 @@ __device__ void forward_solve(const Sys& S, TA* Aug, const int (&posr)[2]) {
 // right-hand sides are taken RC at a time (M is rebuilt once per group) so that the
 // accumulators stay in registers with compile-time indices; the operations of each r_t are
 // the same in any grouping.
-template <typename TA, int MT>
+// TAG separates instances by caller (each instance gets its caller's register budget).
+template <typename TA, int MT, int TAG = 0>
 __device__ double residual(const Sys& S, TA* Aug, const double* XS) {
     constexpr int RC = 4;
     const int lane = threadIdx.x & (WAVE - 1), m = S.m, ld = S.ld, T = S.T;
@@ -1106,21 +1108,35 @@ __host__ __device__ inline int lu_pick(int m, int T) {
     return -1;
 }
 
-// LDS of one team, in doubles: [R0][BS: m T][dg: m][amax: 8][umax: 8] then ints
-// [perm: m][pos: 16 RT][flag: 4]. R0 holds the staged coordinates (5n) during assembly, the
-// panel, two multiplier buffers (each [8][16 RT + 4]) and the U-row buffer ([8][16 W NCT + 4])
-// during the factorisation, and the strict upper triangle by columns (m(m-1)/2) for back
-// substitution. The panel buffers are column-major with a padded stride, so that the lanes of
-// one column, and the four columns one wave reads at once, fall on different banks.
-__host__ __device__ inline size_t lu_r0(int W, int RT, int NCT, int m, int n) {
-    const size_t a = (size_t)5 * n, b = (size_t)24 * (16 * RT + 4) + (size_t)8 * (16 * W * NCT + 4), c = (size_t)m * (m - 1) / 2;
+// Right-hand sides per wave in one back-substitution pass: LU_QC, or LU_QA when the
+// right-hand-side buffer shares R0 with the packed U (see lu_rhs_in_r0).
+constexpr int LU_QC = 4;
+__host__ __device__ constexpr int lu_qa(int W) { return W == 1 ? 4 : W == 2 ? 2 : 1; }
+
+// The right-hand-side buffer BS lives inside R0 (after the staged coordinates) when one
+// back-substitution pass of LU_QA right-hand sides per wave covers all T: the waves then read
+// their right-hand sides into registers before the packed U is written over them.
+__host__ __device__ inline bool lu_rhs_in_r0(int W, int T) { return T <= W * lu_qa(W); }
+
+// LDS of one team, in doubles: [R0][BS: m T, unless inside R0][dg: m][amax: 8][umax: 8], then
+// [flag: 4 int] and 16-bit row numbers [perm: m][pos: 16 RT]. R0 holds the staged coordinates
+// (5n) and the assembled right-hand sides during assembly, the panel, two multiplier buffers
+// (each [8][16 RT + 4]) and the U-row buffer ([8][16 W NCT + 4]) during the factorisation, and
+// the strict upper triangle by columns (m(m-1)/2) for back substitution. The panel buffers are
+// column-major with a padded stride, so that the lanes of one column, and the four columns one
+// wave reads at once, fall on different banks. With BS inside R0 and 16-bit row tables the
+// largest degree-2 system (m = 111, 4 right-hand sides) takes 50.4 KB: three workgroups fit in
+// the 160 KB of a CU (a workgroup's LDS is allocated in 2 KB steps).
+__host__ __device__ inline size_t lu_r0(int W, int RT, int NCT, int m, int n, int T) {
+    const size_t a = (size_t)5 * n + (lu_rhs_in_r0(W, T) ? (size_t)m * T : 0),
+                 b = (size_t)24 * (16 * RT + 4) + (size_t)8 * (16 * W * NCT + 4), c = (size_t)m * (m - 1) / 2;
     const size_t ab = a > b ? a : b;
     return ab > c ? ab : c;
 }
 __host__ __device__ inline size_t lu_team_bytes(int W, int RT, int NCT, int m, int n, int T) {
-    const size_t d = lu_r0(W, RT, NCT, m, n) + (size_t)m * T + m + 16;
-    const size_t i = (size_t)m + 16 * RT + 4;
-    return (d * sizeof(double) + i * sizeof(int) + 15) / 16 * 16;
+    const size_t d = lu_r0(W, RT, NCT, m, n, T) + (lu_rhs_in_r0(W, T) ? 0 : (size_t)m * T) + m + 16;
+    const size_t i = 4 * sizeof(int) + ((size_t)m + 16 * RT) * sizeof(short);
+    return (d * sizeof(double) + i + 15) / 16 * 16;
 }
 
 // The value of x, hidden from the optimiser: lane-dependent indices derived from it are
@@ -1145,7 +1161,9 @@ __device__ __forceinline__ void team_sync() {
 // nodes and m = n + np unknowns, one per team. One-wave teams without a stencil return at
 // once; larger teams without one repeat the first team's solve and write nothing, so that
 // every team passes the same workgroup barriers.
-template <int W, int RT, int NCT>
+// OCC is the waves per SIMD of the calling kernel: the instances of the two kernels are kept
+// apart so that each is compiled to its kernel's register budget.
+template <int W, int RT, int NCT, int OCC>
 __device__ __attribute__((noinline)) void lu_team(const WParams& P, int start, int count, int n, int m, unsigned char* smem) {
     constexpr int MP = 16 * RT, NP = 16 * W * NCT, RR = (MP + WAVE - 1) / WAVE;
     constexpr int PS = MP + 4, US = NP + 4;  // strides of the panel / multiplier and U-row buffers
@@ -1174,14 +1192,15 @@ __device__ __attribute__((noinline)) void lu_team(const WParams& P, int start, i
     S.perm = nullptr;
     // team LDS
     unsigned char* base = smem + (size_t)team * lu_team_bytes(W, RT, NCT, m, n, T);
+    const bool rhs_in_r0 = lu_rhs_in_r0(W, T);
     double* R0 = (double*)base;
-    double* BS = R0 + lu_r0(W, RT, NCT, m, n);
-    double* dg = BS + (size_t)m * T;
+    double* BS = rhs_in_r0 ? R0 + 5 * n : R0 + lu_r0(W, RT, NCT, m, n, T);
+    double* dg = R0 + lu_r0(W, RT, NCT, m, n, T) + (rhs_in_r0 ? 0 : (size_t)m * T);
     double* amaxw = dg + m;
     double* umaxw = amaxw + 8;
-    int* perm = (int*)(umaxw + 8);
-    int* pos = perm + m;
-    int* flag = pos + MP;
+    int* flag = (int*)(umaxw + 8);
+    short* perm = (short*)(flag + 4);
+    short* pos = perm + m;
     double* Pbuf = R0;                  // [KB][PS]
     double* Lbuf = R0 + KB * PS;        // [2][KB][PS]
     double* U12 = R0 + 3 * KB * PS;     // [KB][US]
@@ -1431,51 +1450,55 @@ __device__ __attribute__((noinline)) void lu_team(const WParams& P, int start, i
         if (W == 1 || LU_WAVES == W) return;  // the only team of its barriers
         writer = false;  // other teams share the workgroup barriers below: run on, write nothing
     }
-    // U part (columns right of each row's pivot step) and eliminated right-hand sides by pivot order
+    // U part (columns right of each row's pivot step) and eliminated right-hand sides by pivot
+    // order. With BS inside R0 the right-hand sides go first: every wave reads its own into
+    // registers before the packed U is written over them.
     lane = opaque_v(lane);
     rl = lane & 15;
     cg = lane >> 4;
-    double um = 0.0;
+    auto store_rows = [&](bool u_part, bool rhs_part) -> double {
+        double um = 0.0;
 #pragma unroll
-    for (int rt = 0; rt < RT; rt++) {
-        const int r = 16 * rt + rl;
-        if (r >= m) continue;
-        const int i = pos[r];
+        for (int rt = 0; rt < RT; rt++) {
+            const int r = 16 * rt + rl;
+            if (r >= m) continue;
+            const int i = pos[r];
 #pragma unroll
-        for (int sl = 0; sl < NCT; sl++) {
-            const int c0 = 16 * (wt + W * sl) + cg;
+            for (int sl = 0; sl < NCT; sl++) {
+                const int c0 = 16 * (wt + W * sl) + cg;
 #pragma unroll
-            for (int v = 0; v < 4; v++) {
-                const int c = c0 + 4 * v;
-                const double val = acc[rt][sl][v];
-                if (c < m) {
-                    if (c >= i) um = fmax(um, fabs(val));
-                    if (c > i) Upk[(size_t)c * (c - 1) / 2 + i] = val;
-                } else if (c < m + T) {
-                    BS[i * T + (c - m)] = val;
+                for (int v = 0; v < 4; v++) {
+                    const int c = c0 + 4 * v;
+                    const double val = acc[rt][sl][v];
+                    if (c < m) {
+                        if (u_part) {
+                            if (c >= i) um = fmax(um, fabs(val));
+                            if (c > i) Upk[(size_t)c * (c - 1) / 2 + i] = val;
+                        }
+                    } else if (c < m + T) {
+                        if (rhs_part) BS[i * T + (c - m)] = val;
+                    }
                 }
             }
         }
-    }
-    um = wave_max(um);
-    if (lane == 0) umaxw[wt] = um;
-    team_sync<W>();
-    // back substitution, right-hand sides t = t0 + wt + W q (chunks of QC per wave); lane i
-    // holds unknown i (pivot order)
-    constexpr int QC = 4;
-    lane = opaque_v(lane);
-    tt = wt * WAVE + lane;
-    for (int t0 = 0; t0 < T; t0 += W * QC) {
-        double b[QC][RR];
+        return um;
+    };
+    // Back substitution of the right-hand sides t = t0 + wt + W q, q < Q: b[q][h] holds
+    // unknown lane + 64 h (pivot order); then the weights out.
+    auto load_rhs = [&](auto qtag, auto& b, int t0) {
+        constexpr int Q = decltype(qtag)::value;
 #pragma unroll
         for (int h = 0; h < RR; h++) {
             const int i = lane + WAVE * h;
 #pragma unroll
-            for (int q = 0; q < QC; q++) {
+            for (int q = 0; q < Q; q++) {
                 const int t = t0 + wt + W * q;
                 b[q][h] = (t < T && i < m) ? BS[i * T + t] : 0.0;
             }
         }
+    };
+    auto back_sub = [&](auto qtag, auto& b, int t0) {
+        constexpr int Q = decltype(qtag)::value;
         // column k of U (rows < k) and the pivot u_kk are loaded one step ahead
         double ucn[RR], ukn = dg[m - 1];
 #pragma unroll
@@ -1493,7 +1516,7 @@ __device__ __attribute__((noinline)) void lu_team(const WParams& P, int start, i
                 ukn = dg[k - 1];
             }
 #pragma unroll
-            for (int q = 0; q < QC; q++) {
+            for (int q = 0; q < Q; q++) {
                 if (t0 + wt + W * q >= T) continue;
                 const double bk = readlane_f64((RR == 1 || k < WAVE) ? b[q][0] : b[q][RR - 1], k & (WAVE - 1));
                 const double x = bk / ukk;
@@ -1509,7 +1532,7 @@ __device__ __attribute__((noinline)) void lu_team(const WParams& P, int start, i
         }
         __builtin_amdgcn_wave_barrier();
 #pragma unroll
-        for (int q = 0; q < QC; q++) {
+        for (int q = 0; q < Q; q++) {
             const int t = t0 + wt + W * q;
             if (t >= T) continue;
             const double scale = (P.types[t] == RBFX_HV) ? P.hv_scale : 1.0;
@@ -1517,10 +1540,51 @@ __device__ __attribute__((noinline)) void lu_team(const WParams& P, int start, i
             for (int h = 0; h < RR; h++) {
                 const int i = lane + WAVE * h;
                 if (i < n && writer) P.w[(size_t)t * P.nnz + S.off + i] = b[q][h] * scale;
-                // x by unknown for the diagnostics (this wave's columns of BS: every unknown of
-                // these right-hand sides has been read above)
-                if (P.diag && i < m) BS[i * T + t] = b[q][h];
             }
+        }
+    };
+    // x by unknown into BS, for the diagnostics
+    auto keep_x = [&](auto qtag, auto& b, int t0) {
+        constexpr int Q = decltype(qtag)::value;
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            const int t = t0 + wt + W * q;
+            if (t >= T) continue;
+#pragma unroll
+            for (int h = 0; h < RR; h++) {
+                const int i = lane + WAVE * h;
+                if (i < m) BS[i * T + t] = b[q][h];
+            }
+        }
+    };
+    lane = opaque_v(lane);
+    tt = wt * WAVE + lane;
+    if (rhs_in_r0) {
+        using QA = std::integral_constant<int, lu_qa(W)>;
+        store_rows(false, true);
+        team_sync<W>();
+        double b[QA::value][RR];
+        load_rhs(QA{}, b, 0);
+        team_sync<W>();  // every wave holds its right-hand sides: BS may be overwritten
+        const double um = wave_max(store_rows(true, false));
+        if (lane == 0) umaxw[wt] = um;
+        team_sync<W>();
+        back_sub(QA{}, b, 0);
+        if (P.diag) {        // uniform
+            team_sync<W>();  // every wave has read the packed U under BS
+            keep_x(QA{}, b, 0);
+        }
+    } else {
+        using QC = std::integral_constant<int, LU_QC>;
+        const double um = wave_max(store_rows(true, true));
+        if (lane == 0) umaxw[wt] = um;
+        team_sync<W>();
+        for (int t0 = 0; t0 < T; t0 += W * LU_QC) {
+            double b[LU_QC][RR];
+            load_rhs(QC{}, b, t0);
+            back_sub(QC{}, b, t0);
+            // each wave overwrites only the columns of BS it has read itself
+            if (P.diag) keep_x(QC{}, b, t0);
         }
     }
     if (tt == 0 && writer) P.fail_flags[s] = 0;
@@ -1546,7 +1610,7 @@ __device__ __attribute__((noinline)) void lu_team(const WParams& P, int start, i
         am = fmax(am, amaxw[w]);
         umx = fmax(umx, umaxw[w]);
     }
-    const double be = residual<double, MAX_T>(S, (double*)nullptr, BS);
+    const double be = residual<double, MAX_T, OCC>(S, (double*)nullptr, BS);
     if (lane == 0) {
         P.diag[4 * (size_t)s] = be;
         P.diag[4 * (size_t)s + 1] = umx / am;
@@ -1556,9 +1620,12 @@ __device__ __attribute__((noinline)) void lu_team(const WParams& P, int start, i
 }
 
 // Work unit: {first position in P.order, stencils (<= teams of the layout), layout | n << 8, m}.
+// The narrow kernel (layouts 0-4, tiles of at most 64 doubles per lane) is compiled for three
+// waves per SIMD (168 VGPRs); the wide one (96 doubles of tiles per lane) for two.
 template <bool WIDE>
-__global__ void __launch_bounds__(LU_BLOCK) __attribute__((amdgpu_waves_per_eu(2))) weights_lu_kernel(WParams P, const int4* __restrict__ units, int n_units,
+__global__ void __launch_bounds__(LU_BLOCK) __attribute__((amdgpu_waves_per_eu(WIDE ? 2 : 3))) weights_lu_kernel(WParams P, const int4* __restrict__ units, int n_units,
                                                              int* __restrict__ counter) {
+    constexpr int OCC = WIDE ? 2 : 3;
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ int s_unit;
     for (;;) {
@@ -1569,13 +1636,13 @@ __global__ void __launch_bounds__(LU_BLOCK) __attribute__((amdgpu_waves_per_eu(2
         const int4 U = units[u];
         const int lay = U.z & 0xff, n = U.z >> 8, m = U.w;
         switch (lay) {
-            case 0: lu_team<1, 2, 2>(P, U.x, U.y, n, m, smem); break;
-            case 1: lu_team<2, 3, 2>(P, U.x, U.y, n, m, smem); break;
-            case 2: lu_team<2, 4, 2>(P, U.x, U.y, n, m, smem); break;
-            case 3: lu_team<4, 6, 2>(P, U.x, U.y, n, m, smem); break;
-            case 4: lu_team<4, 8, 2>(P, U.x, U.y, n, m, smem); break;
+            case 0: lu_team<1, 2, 2, OCC>(P, U.x, U.y, n, m, smem); break;
+            case 1: lu_team<2, 3, 2, OCC>(P, U.x, U.y, n, m, smem); break;
+            case 2: lu_team<2, 4, 2, OCC>(P, U.x, U.y, n, m, smem); break;
+            case 3: lu_team<4, 6, 2, OCC>(P, U.x, U.y, n, m, smem); break;
+            case 4: lu_team<4, 8, 2, OCC>(P, U.x, U.y, n, m, smem); break;
             default:
-                if constexpr (WIDE) lu_team<4, 8, 3>(P, U.x, U.y, n, m, smem);
+                if constexpr (WIDE) lu_team<4, 8, 3, OCC>(P, U.x, U.y, n, m, smem);
                 break;
         }
         __syncthreads();  // the team LDS and s_unit are free for the next unit

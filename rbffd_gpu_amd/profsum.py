@@ -1,13 +1,15 @@
 """Summarise rocprofv3 result databases (one kernel-trace pass and any number of --pmc passes
 over the same program) into the markdown tables kept under profiles/.
 
-usage: python -m rbffd_gpu_amd.profsum <prof_dir> <algorithmic_bytes_json> [kernel-substring ...]
+usage: python -m rbffd_gpu_amd.profsum [--per-dispatch] <prof_dir> <algorithmic_bytes_json> [kernel-substring ...]
 <prof_dir> holds one subfolder per pass, each with the .db that `rocprofv3 -d <prof_dir>/<pass>
 -o <pass>` wrote; the pass named kt is the kernel trace (durations, registers, LDS), all others
 are counter passes. <algorithmic_bytes_json> maps kernel-name substrings to algorithmic bytes per
 launch ("{}" for none). Traffic calibration: the plain copy kernel (stream_copy_kernel) in the
 same program copies 1 GiB; its FETCH_SIZE / WRITE_SIZE per dispatch give the bytes per counted
-KiB, which scale the other kernels' counters into HBM bytes per launch.
+KiB, which scale the other kernels' counters into HBM bytes per launch. With --per-dispatch every
+dispatch of a kernel gets its own row, named `<kernel> #k` for its k-th dispatch (in launch order),
+instead of one row averaged over the kernel's dispatches.
 """
 import glob
 import json
@@ -33,7 +35,7 @@ def want(name, pats):
     return not pats or any(p in name for p in pats) or "stream_copy" in name
 
 
-def summarize(d, alg, pats):
+def summarize(d, alg, pats, per_dispatch=False):
     """Markdown tables for the passes under d (see the module docstring)."""
     out = []
     dur = {}
@@ -43,17 +45,34 @@ def summarize(d, alg, pats):
         out.append("")
         out.append("| kernel | launches | avg us | min us | max us | VGPR | AGPR | LDS B | wg | grid |")
         out.append("|---|---|---|---|---|---|---|---|---|---|")
-        for name, cnt, avg, mn, mx, v, a, lds, wg, gr in c.execute(
+        if per_dispatch:
+            rows, seen = [], defaultdict(int)
+            for name, t, v, a, lds, wg, gr in c.execute(
+                    "select name, end-start, vgpr_count, accum_vgpr_count, lds_size, workgroup_x, grid_x from kernels "
+                    "order by start"):
+                seen[name] += 1
+                rows.append((f"{short(name)} #{seen[name]}", name, 1, t, t, t, v, a, lds, wg, gr))
+            rows.sort(key=lambda r: -r[3])
+        else:
+            rows = [(short(r[0]),) + tuple(r) for r in c.execute(
                 "select name, count(*), avg(end-start), min(end-start), max(end-start), vgpr_count, accum_vgpr_count, "
-                "lds_size, workgroup_x, grid_x from kernels group by name order by sum(end-start) desc"):
+                "lds_size, workgroup_x, grid_x from kernels group by name order by sum(end-start) desc")]
+        for key, name, cnt, avg, mn, mx, v, a, lds, wg, gr in rows:
             if not want(name, pats):
                 continue
-            dur[short(name)] = avg / 1e3
-            out.append(f"| `{short(name)}` | {cnt} | {avg / 1e3:.1f} | {mn / 1e3:.1f} | {mx / 1e3:.1f} | {v} | {a} | {lds} | {wg} | {gr} |")
+            dur[key] = avg / 1e3
+            out.append(f"| `{key}` | {cnt} | {avg / 1e3:.1f} | {mn / 1e3:.1f} | {mx / 1e3:.1f} | {v} | {a} | {lds} | {wg} | {gr} |")
     cnt = defaultdict(dict)
     for p in sorted(os.path.basename(x) for x in glob.glob(os.path.join(d, "*")) if os.path.isdir(x) and not x.endswith("kt")):
         c = db(d, p)
         if not c:
+            continue
+        if per_dispatch:
+            seen = defaultdict(int)
+            for name, ctr, val in c.execute("select kernel_name, counter_name, sum(value) from counters_collection "
+                                            "group by dispatch_id, counter_name order by dispatch_id, counter_name"):
+                seen[name, ctr] += 1  # each dispatch reports every counter of the pass once
+                cnt[f"{short(name)} #{seen[name, ctr]}"][ctr] = val
             continue
         for name, ctr, val in c.execute("select kernel_name, counter_name, avg(v) from (select kernel_name, counter_name, "
                                         "dispatch_id, sum(value) v from counters_collection group by dispatch_id, counter_name) "
@@ -94,4 +113,8 @@ def summarize(d, alg, pats):
 
 
 if __name__ == "__main__":
-    print(summarize(sys.argv[1], json.loads(sys.argv[2]), sys.argv[3:]))
+    args = sys.argv[1:]
+    per = args[:1] == ["--per-dispatch"]
+    if per:
+        args = args[1:]
+    print(summarize(args[0], json.loads(args[1]), args[2:], per_dispatch=per))
