@@ -1082,7 +1082,7 @@ typedef double d4v __attribute__((ext_vector_type(4)));
 
 constexpr int LU_BLOCK = 256;   // threads per workgroup (4 waves)
 constexpr int LU_WAVES = LU_BLOCK / WAVE;
-constexpr int LU_NLAYOUT = 6;   // layout 5 (three column tiles per wave) only in the wide kernel
+constexpr int LU_NLAYOUT = 7;   // layout 6 (three column tiles per wave) only in the wide kernel
 
 struct LuLayout {
     int W, RT, NCT;
@@ -1095,7 +1095,8 @@ __host__ __device__ constexpr LuLayout lu_layout(int id) {
          : id == 1 ? LuLayout{2, 3, 2}
          : id == 2 ? LuLayout{2, 4, 2}
          : id == 3 ? LuLayout{4, 6, 2}
-         : id == 4 ? LuLayout{4, 8, 2}
+         : id == 4 ? LuLayout{4, 7, 2}   // m <= 112: the degree-2 systems of 101-node stencils
+         : id == 5 ? LuLayout{4, 8, 2}
                    : LuLayout{4, 8, 3};
 }
 
@@ -1156,38 +1157,56 @@ __device__ __forceinline__ void team_sync() {
 }
 
 // Not inlined: each layout then gets its own register allocation (inlined into one kernel
-// the five layouts spill hundreds of VGPRs).
+// the layouts spill hundreds of VGPRs).
 // Solves the stencils order[start .. start + count) (count <= number of teams), each of n
 // nodes and m = n + np unknowns, one per team. One-wave teams without a stencil return at
 // once; larger teams without one repeat the first team's solve and write nothing, so that
 // every team passes the same workgroup barriers.
 // OCC is the waves per SIMD of the calling kernel: the instances of the two kernels are kept
 // apart so that each is compiled to its kernel's register budget.
+// Arguments of a device function arrive in VGPRs, and values loaded through P are per lane to
+// the compiler: the sizes, counts and flags are made wave-uniform (readfirstlane) on entry, so
+// that the branches and loop bounds on them are scalar branches instead of exec-masked
+// regions around every tile operation.
+__device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ double uni(double x) {
+    const unsigned long long u = __double_as_longlong(x);
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)u), hi = __builtin_amdgcn_readfirstlane((unsigned)(u >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 template <int W, int RT, int NCT, int OCC>
 __device__ __attribute__((noinline)) void lu_team(const WParams& P, int start, int count, int n, int m, unsigned char* smem) {
     constexpr int MP = 16 * RT, NP = 16 * W * NCT, RR = (MP + WAVE - 1) / WAVE;
     constexpr int PS = MP + 4, US = NP + 4;  // strides of the panel / multiplier and U-row buffers
     constexpr int KB = 8;                    // panel width
-    const int wv = threadIdx.x / WAVE;
+    start = uni(start);
+    count = uni(count);
+    n = uni(n);
+    m = uni(m);
+    const int wv = uni((int)threadIdx.x / WAVE);
     int lane = opaque_v(threadIdx.x & (WAVE - 1));
     const int team = wv / W, wt = wv - team * W;
     if (W == 1 && team >= count) return;
     bool writer = team < count;
-    const int s = P.order[start + (writer ? team : 0)];
-    const int T = P.T;
+    const int s = uni(P.order[start + (writer ? team : 0)]);
+    const int T = uni(P.T);
+    const bool diag = uni(P.diag != nullptr ? 1 : 0) != 0;
+    const int sph_basis = uni(P.sph);
     int rl = lane & 15, cg = lane >> 4;
     Sys S;
     S.P = &P;
     S.s = s;
-    S.off = P.st_off[s];
+    S.off = uni(P.st_off[s]);
     S.n = n;
     S.m = m;
     S.T = T;
     S.ld = 0;
-    S.eps = P.eps[s];
+    S.eps = uni(P.eps[s]);
     S.e2 = S.eps * S.eps;
-    S.sph = false;
-    for (int t = 0; t < T; t++) S.sph |= (P.types[t] == RBFX_LAMBDA || P.types[t] == RBFX_THETA);
+    int sph = 0;
+    for (int t = 0; t < T; t++) sph |= (P.types[t] == RBFX_LAMBDA || P.types[t] == RBFX_THETA);
+    S.sph = uni(sph) != 0;
     S.XS = S.BS = nullptr;
     S.perm = nullptr;
     // team LDS
@@ -1240,7 +1259,7 @@ __device__ __attribute__((noinline)) void lu_team(const WParams& P, int start, i
                 double val = 0.0;
                 if (r < m) {
                     if (c < m) {
-                        val = m_entry(r, c, n, S.e2, S.X, P.sph);
+                        val = m_entry(r, c, n, S.e2, S.X, sph_basis);
                         amax = fmax(amax, fabs(val));
                     } else if (c < m + T) {
                         val = BS[r * T + (c - m)];
@@ -1374,7 +1393,7 @@ __device__ __attribute__((noinline)) void lu_team(const WParams& P, int start, i
 #pragma unroll
         for (int q = 0; q < KB; q++) {
             if (q >= npn) break;
-            const int p = perm[kp + q];
+            const int p = uni((int)perm[kp + q]);
             if (rl == (p & 15)) {
 #pragma unroll
                 for (int sl = 0; sl < NCT; sl++) {
@@ -1402,7 +1421,7 @@ __device__ __attribute__((noinline)) void lu_team(const WParams& P, int start, i
 #pragma unroll
                 for (int q = 1; q < KB; q++)
                     if (q < npn) {
-                        const int pq = perm[kp + q];
+                        const int pq = uni((int)perm[kp + q]);
 #pragma unroll
                         for (int jj = 0; jj < q; jj++) u[q] = u[q] - Lb[jj * PS + pq] * u[jj];
                     }
@@ -1439,7 +1458,7 @@ __device__ __attribute__((noinline)) void lu_team(const WParams& P, int start, i
             }
             if (lane == 0) {
                 P.fail_flags[s] = 1;
-                if (P.diag) {
+                if (diag) {
                     P.diag[4 * (size_t)s] = NAN;
                     P.diag[4 * (size_t)s + 1] = 0;
                     P.diag[4 * (size_t)s + 2] = 0;
@@ -1570,7 +1589,7 @@ __device__ __attribute__((noinline)) void lu_team(const WParams& P, int start, i
         if (lane == 0) umaxw[wt] = um;
         team_sync<W>();
         back_sub(QA{}, b, 0);
-        if (P.diag) {        // uniform
+        if (diag) {
             team_sync<W>();  // every wave has read the packed U under BS
             keep_x(QA{}, b, 0);
         }
@@ -1584,11 +1603,11 @@ __device__ __attribute__((noinline)) void lu_team(const WParams& P, int start, i
             load_rhs(QC{}, b, t0);
             back_sub(QC{}, b, t0);
             // each wave overwrites only the columns of BS it has read itself
-            if (P.diag) keep_x(QC{}, b, t0);
+            if (diag) keep_x(QC{}, b, t0);
         }
     }
     if (tt == 0 && writer) P.fail_flags[s] = 0;
-    if (!P.diag) return;  // uniform
+    if (!diag) return;
     // diagnostics: x by unknown is in BS; the coordinates are restaged over the packed U
     team_sync<W>();  // back substitution has read the packed U
     for (int j = tt; j < n; j += WAVE * W) {
@@ -1620,7 +1639,7 @@ __device__ __attribute__((noinline)) void lu_team(const WParams& P, int start, i
 }
 
 // Work unit: {first position in P.order, stencils (<= teams of the layout), layout | n << 8, m}.
-// The narrow kernel (layouts 0-4, tiles of at most 64 doubles per lane) is compiled for three
+// The narrow kernel (layouts 0-5, tiles of at most 64 doubles per lane) is compiled for three
 // waves per SIMD (168 VGPRs); the wide one (96 doubles of tiles per lane) for two.
 template <bool WIDE>
 __global__ void __launch_bounds__(LU_BLOCK) __attribute__((amdgpu_waves_per_eu(WIDE ? 2 : 3))) weights_lu_kernel(WParams P, const int4* __restrict__ units, int n_units,
@@ -1640,7 +1659,8 @@ __global__ void __launch_bounds__(LU_BLOCK) __attribute__((amdgpu_waves_per_eu(W
             case 1: lu_team<2, 3, 2, OCC>(P, U.x, U.y, n, m, smem); break;
             case 2: lu_team<2, 4, 2, OCC>(P, U.x, U.y, n, m, smem); break;
             case 3: lu_team<4, 6, 2, OCC>(P, U.x, U.y, n, m, smem); break;
-            case 4: lu_team<4, 8, 2, OCC>(P, U.x, U.y, n, m, smem); break;
+            case 4: lu_team<4, 7, 2, OCC>(P, U.x, U.y, n, m, smem); break;
+            case 5: lu_team<4, 8, 2, OCC>(P, U.x, U.y, n, m, smem); break;
             default:
                 if constexpr (WIDE) lu_team<4, 8, 3, OCC>(P, U.x, U.y, n, m, smem);
                 break;
@@ -1800,7 +1820,7 @@ int weights_impl(rbfx_ctx* ctx, int32_t n_stencils, const int32_t* d_st_off, con
             const int id = lu_pick(m, T);
             if (id < 0) return rbfx_fail(ctx, RBFX_E_TOO_LARGE, "weights: system %d x %d with %d RHS has no layout", m, m, T);
             const LuLayout L = lu_layout(id);
-            wide |= id >= 5;
+            wide |= id >= 6;
             const int teams = LU_WAVES / L.W;
             lds = std::max(lds, (size_t)teams * lu_team_bytes(L.W, L.RT, L.NCT, m, n, T));
             for (int i = 0; i < c; i += teams)
