@@ -529,19 +529,23 @@ def main():
     if not args.no_revolution:
         n_rev = int(round(1.0 / dt))
         ends = {}
-        for name, rk in (("fp32", rk32), ("fp64", rk64)):
+        for name, rk in (("fp32", rk32), ("fp32_acc64", rk32a), ("fp64", rk64)):
             rk.set_state(d_u0)
             rk.step(dt, n_rev)
             ends[name] = rk.get_state()
-        r32, r64 = ends["fp32"], ends["fp64"]
+        r32, r32a, r64 = ends["fp32"], ends["fp32_acc64"], ends["fp64"]
         rev = {"steps": n_rev,
                "fp32_rel_deviation_from_fp64": float(np.abs(r32 - r64).max() / np.abs(r64).max()),
                "fp32_rms_rel_deviation_from_fp64": float(np.sqrt(np.mean((r32 - r64) ** 2) / np.mean(r64 ** 2))),
+               "fp32_acc64_rel_deviation_from_fp64": float(np.abs(r32a - r64).max() / np.abs(r64).max()),
+               "fp32_acc64_rms_rel_deviation_from_fp64": float(np.sqrt(np.mean((r32a - r64) ** 2) /
+                                                                       np.mean(r64 ** 2))),
                "max_rel_error_vs_analytic": {k: float(np.abs(v - u0).max() / np.abs(u0).max()) for k, v in ends.items()},
                "l2_rel_error_vs_analytic": {k: float(np.sqrt(np.mean((v - u0) ** 2) / np.mean(u0 ** 2)))
                                             for k, v in ends.items()}}
         log(f"one revolution ({n_rev} steps): deviation FP32 vs FP64 {rev['fp32_rel_deviation_from_fp64']:.2e} (max) "
-            f"{rev['fp32_rms_rel_deviation_from_fp64']:.2e} (rms); error vs analytic max {rev['max_rel_error_vs_analytic']}, "
+            f"{rev['fp32_rms_rel_deviation_from_fp64']:.2e} (rms), with FP64 accumulation "
+            f"{rev['fp32_acc64_rel_deviation_from_fp64']:.2e} (max); error vs analytic max {rev['max_rel_error_vs_analytic']}, "
             f"l2 {rev['l2_rel_error_vs_analytic']}")
 
     # ---- dominant kernel: the fused stage kernel, timed per launch ---------------------------
