@@ -119,9 +119,10 @@ __global__ void __launch_bounds__(256) ell_spmm_kernel(const int32_t* __restrict
 // (rbfx_ell_build_local). The tile's distinct neighbour records are copied into LDS once, then
 // every row gathers from there through 16-bit local indices; the summation order is that of
 // ell_spmm_kernel, so the outputs are the same bit for bit. MX: FP32 multiply-adds on the
-// matrix cores (16 fields on 4 lanes per row).
+// matrix cores (16 fields on 4 lanes per row). FP32-accumulation instances: six waves per SIMD
+// (three 512-thread blocks per CU), as in rk4_stage_lds_kernel.
 template <typename WT, typename UT, typename AT, int NOPS, int FPT, int UB, int SG, int PD, bool MX>
-__global__ void __launch_bounds__(512) ell_spmm_lds_kernel(const int32_t* __restrict__ loc_pos,
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu((sizeof(UT) == 4 && sizeof(AT) == 4) ? 6 : 1))) ell_spmm_lds_kernel(const int32_t* __restrict__ loc_pos,
                                                            const int32_t* __restrict__ loc_off,
                                                            const int32_t* __restrict__ loc_nodes,
                                                            const uint16_t* __restrict__ loc_idx,
@@ -133,19 +134,27 @@ __global__ void __launch_bounds__(512) ell_spmm_lds_kernel(const int32_t* __rest
     const int tile = xcd_tile(blockIdx.x, nb);
     if (tile >= n_tiles) return;  // uniform over the block
     const int k0 = loc_off[tile];
-    ell_stage_tile<UT>(u, F, loc_nodes + k0, loc_off[tile + 1] - k0, lds);
-    __syncthreads();
     const int t = threadIdx.x;
     const int rl = t / SG;
     const int p = loc_pos[tile] + rl;
-    if (p >= loc_pos[tile + 1]) return;  // after the only barrier
-    const int row = ell_row_at(order, p);
+    const bool live = p < loc_pos[tile + 1];
     const int fofs = (t - rl * SG) * FPT;
     AT d[NOPS][FPT];
-    if constexpr (MX)
-        ell_gather_dot_lds_mfma<NOPS, UB, PD>(loc_idx, (const float*)w_t, (const float*)lds, p, width, fofs, t - rl * SG, d);
-    else
+    if constexpr (MX) {
+        // the first index and weight batches are requested before the staging
+        LdsMfmaGather<NOPS, UB, PD> g;
+        if (live) g.start(loc_idx, (const float*)w_t, p, width, t - rl * SG);
+        ell_stage_tile<UT>(u, F, loc_nodes + k0, loc_off[tile + 1] - k0, lds);
+        __syncthreads();
+        if (!live) return;  // after the only barrier
+        g.finish((const float*)lds, fofs, d);
+    } else {
+        ell_stage_tile<UT>(u, F, loc_nodes + k0, loc_off[tile + 1] - k0, lds);
+        __syncthreads();
+        if (!live) return;  // after the only barrier
         ell_gather_dot_lds<WT, UT, AT, NOPS, FPT, UB, SG, PD>(loc_idx, w_t, lds, p, width, fofs, t - rl * SG, d);
+    }
+    const int row = ell_row_at(order, p);
 #pragma unroll
     for (int o = 0; o < NOPS; o++) {
         UT r[FPT];

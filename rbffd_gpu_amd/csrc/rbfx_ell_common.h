@@ -584,30 +584,26 @@ __device__ __forceinline__ void ell_gather_dot_lds(const uint16_t* __restrict__ 
 // group and broadcast (DPP). The k order per accumulator is the same as in the VALU cores.
 // MFMA reads operands across the lanes of a block only, and a row's 4 lanes are active or
 // inactive together, so rows that exited early do not disturb the others.
+//
+// The core is split so that a kernel can issue the first PD batches of index and weight loads
+// (which do not read LDS) before its staging barrier: start() issues them, finish() runs the
+// pipelined loop over the staged records.
 template <int NOPS, int UB, int PD>
-__device__ __forceinline__ void ell_gather_dot_lds_mfma(const uint16_t* __restrict__ loc_idx,
-                                                        const float* __restrict__ w_t, const float* lds, int p,
-                                                        int width, int fofs, int q, float (&acc)[NOPS][4]) {
+struct LdsMfmaGather {
     typedef float f4 __attribute__((ext_vector_type(4)));
-    constexpr int G = 4, L = UB / G, F = 16;
+    static constexpr int G = 4, L = UB / G, F = 16;
     static_assert(UB % G == 0 && NOPS <= 4, "batch of whole groups, at most 4 operators");
-    f4 D[4];
-#pragma unroll
-    for (int g = 0; g < 4; g++) D[g] = f4{0.f, 0.f, 0.f, 0.f};
-    const size_t base = ell_row_base(p, width);
-    const uint16_t* lip = loc_idx + base;
-    // lanes q >= NOPS load operator 0's weights: their A values only reach row q of each 4x4
-    // block, i.e. the accumulators of operator q, which are not returned
-    const float* wq = w_t + base * NOPS + (q < NOPS ? q : 0);
-    const size_t wstride = (size_t)ELL_TILE * NOPS;
-    const float* lb = lds + fofs;
-    const int nbatch = (width + UB - 1) / UB;
+    const uint16_t* lip;
+    const float* wq;
+    size_t wstride;
+    int width, q;
     int nbl[PD][L];
     float wl[PD][UB];
+
     // Loads of the batch starting at slot jb. Slots past the row's width read the clamped last
     // slot (in range); the consumer masks them. No branch and no use of the loaded values
     // here, so PD batches of loads stay in flight.
-    auto own = [&](int jb, int (&nb)[L], float (&w)[UB]) {
+    __device__ __forceinline__ void own(int jb, int (&nb)[L], float (&w)[UB]) const {
 #pragma unroll
         for (int l = 0; l < L; l++) {
             const int j = jb + l * G + q;
@@ -615,47 +611,77 @@ __device__ __forceinline__ void ell_gather_dot_lds_mfma(const uint16_t* __restri
         }
 #pragma unroll
         for (int k = 0; k < UB; k++) w[k] = __builtin_nontemporal_load(wq + (size_t)min(jb + k, width - 1) * wstride);
-    };
-    // MASK: slots at or past the width get weight 0 (0 * a finite record: the accumulators do
-    // not change)
-    auto consume = [&](int (&nb)[L], float (&w)[UB], bool mask, int jb) {
-        float v[UB][4];
+    }
+
+    // Row p (processing position) of the operator set; q = lane index inside the row's group.
+    __device__ __forceinline__ void start(const uint16_t* __restrict__ loc_idx, const float* __restrict__ w_t, int p,
+                                          int width_, int q_) {
+        width = width_;
+        q = q_;
+        const size_t base = ell_row_base(p, width);
+        lip = loc_idx + base;
+        // lanes q >= NOPS load operator 0's weights: their A values only reach row q of each
+        // 4x4 block, i.e. the accumulators of operator q, which are not returned
+        wq = w_t + base * NOPS + (q < NOPS ? q : 0);
+        wstride = (size_t)ELL_TILE * NOPS;
+        StaticFor<0, PD>::run([&](auto S) { own(decltype(S)::value * UB, nbl[decltype(S)::value], wl[decltype(S)::value]); });
+    }
+
+    __device__ __forceinline__ void finish(const float* lds, int fofs, float (&acc)[NOPS][4]) {
+        f4 D[4];
 #pragma unroll
-        for (int l = 0; l < L; l++)
-            StaticFor<0, G>::run([&](auto S2) {
-                constexpr int s2 = decltype(S2)::value;
-                const int li = bcast_group<G, s2>(nb[l]);
-                load_pack<float, 4>(lb + li * F, v[l * G + s2]);
+        for (int g = 0; g < 4; g++) D[g] = f4{0.f, 0.f, 0.f, 0.f};
+        const float* lb = lds + fofs;
+        const int nbatch = (width + UB - 1) / UB;
+        // MASK: slots at or past the width get weight 0 (0 * a finite record: the accumulators
+        // do not change)
+        auto consume = [&](int (&nb)[L], float (&w)[UB], bool mask, int jb) {
+            float v[UB][4];
+#pragma unroll
+            for (int l = 0; l < L; l++)
+                StaticFor<0, G>::run([&](auto S2) {
+                    constexpr int s2 = decltype(S2)::value;
+                    const int li = bcast_group<G, s2>(nb[l]);
+                    load_pack<float, 4>(lb + li * F, v[l * G + s2]);
+                });
+#pragma unroll
+            for (int k = 0; k < UB; k++) {
+                const float a = mask ? w[k] * (jb + k < width ? 1.f : 0.f) : w[k];
+#pragma unroll
+                for (int g = 0; g < 4; g++) D[g] = __builtin_amdgcn_mfma_f32_4x4x1f32(a, v[k][g], D[g], 0, 0, 0);
+            }
+        };
+        // the pipelined loop takes whole batches only, a multiple of PD of them; the rest (at
+        // most PD - 1 whole batches and the partial one) are already loaded when it ends
+        const int nfull = (width / UB) / PD * PD;
+        for (int b0 = 0; b0 < nfull; b0 += PD) {
+            StaticFor<0, PD>::run([&](auto S) {
+                constexpr int s = decltype(S)::value;
+                consume(nbl[s], wl[s], false, 0);
+                // keep the reloads after the last use of the registers they overwrite: hoisted
+                // above it they need fresh registers and loop-end copies that wait for every load
+                __builtin_amdgcn_sched_barrier(0);
+                own((b0 + s + PD) * UB, nbl[s], wl[s]);
             });
-#pragma unroll
-        for (int k = 0; k < UB; k++) {
-            const float a = mask ? w[k] * (jb + k < width ? 1.f : 0.f) : w[k];
-#pragma unroll
-            for (int g = 0; g < 4; g++) D[g] = __builtin_amdgcn_mfma_f32_4x4x1f32(a, v[k][g], D[g], 0, 0, 0);
         }
-    };
-    StaticFor<0, PD>::run([&](auto S) { own(decltype(S)::value * UB, nbl[decltype(S)::value], wl[decltype(S)::value]); });
-    // the pipelined loop takes whole batches only, a multiple of PD of them; the rest (at most
-    // PD - 1 whole batches and the partial one) are already loaded when it ends
-    const int nfull = (width / UB) / PD * PD;
-    for (int b0 = 0; b0 < nfull; b0 += PD) {
         StaticFor<0, PD>::run([&](auto S) {
             constexpr int s = decltype(S)::value;
-            consume(nbl[s], wl[s], false, 0);
-            // keep the reloads after the last use of the registers they overwrite: hoisted
-            // above it they need fresh registers and loop-end copies that wait for every load
-            __builtin_amdgcn_sched_barrier(0);
-            own((b0 + s + PD) * UB, nbl[s], wl[s]);
+            if (nfull + s < nbatch) consume(nbl[s], wl[s], true, (nfull + s) * UB);
         });
+#pragma unroll
+        for (int o = 0; o < NOPS; o++)
+#pragma unroll
+            for (int g = 0; g < 4; g++) acc[o][g] = D[g][o];
     }
-    StaticFor<0, PD>::run([&](auto S) {
-        constexpr int s = decltype(S)::value;
-        if (nfull + s < nbatch) consume(nbl[s], wl[s], true, (nfull + s) * UB);
-    });
-#pragma unroll
-    for (int o = 0; o < NOPS; o++)
-#pragma unroll
-        for (int g = 0; g < 4; g++) acc[o][g] = D[g][o];
+};
+
+template <int NOPS, int UB, int PD>
+__device__ __forceinline__ void ell_gather_dot_lds_mfma(const uint16_t* __restrict__ loc_idx,
+                                                        const float* __restrict__ w_t, const float* lds, int p,
+                                                        int width, int fofs, int q, float (&acc)[NOPS][4]) {
+    LdsMfmaGather<NOPS, UB, PD> g;
+    g.start(loc_idx, w_t, p, width, q);
+    g.finish(lds, fofs, acc);
 }
 
 // Maps a hardware block index to a logical tile so that each of the 8 XCDs (which receive

@@ -94,28 +94,44 @@ __global__ void __launch_bounds__(256) rk4_stage_kernel(StageArgs a) {
 // tiles with 4 lanes per row); the tile's distinct neighbour records are copied into LDS
 // first, then every row gathers from there.
 // MX: FP32 multiply-adds on the matrix cores (16 fields, 4 lanes per row).
+// The FP32-accumulation instances are compiled for six waves per SIMD (at most 80 VGPRs):
+// three 512-thread blocks per CU, as many as 48 KB of staged records allow; at the 92 VGPRs the
+// compiler chose unconstrained, only two fitted (FP32 stage at N = 1M, n = 50: 271 -> 266 us).
 template <typename WT, typename UT, typename AT, int NOPS, int FPT, int UB, int SG, int PD, bool MX = false>
-__global__ void __launch_bounds__(512) rk4_stage_lds_kernel(StageArgs a) {
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu((sizeof(UT) == 4 && sizeof(AT) == 4) ? 6 : 1))) rk4_stage_lds_kernel(StageArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     UT* lds = reinterpret_cast<UT*>(smem);
     const int tile = xcd_tile(blockIdx.x, a.nb);
     if (tile >= a.n_tiles) return;  // uniform over the block
     const int t = threadIdx.x;
     const int k0 = a.loc_off[tile];
-    ell_stage_tile<UT>((const UT*)a.xin, a.F, a.loc_nodes + k0, a.loc_off[tile + 1] - k0, lds);
-    __syncthreads();
     const int rl = t / SG;
     const int p = a.loc_pos[tile] + rl;
-    if (p >= a.loc_pos[tile + 1]) return;  // after the only barrier; tiles hold <= lt rows
-    const int row = ell_row_at(a.order, p);
-    const int fofs = (t - rl * SG) * FPT;
-    AT d[NOPS][FPT];
-    if constexpr (MX)
-        ell_gather_dot_lds_mfma<NOPS, UB, PD>(a.loc_idx, (const float*)a.w_t, lds, p, a.width, fofs, t - rl * SG, d);
-    else
+    const bool live = p < a.loc_pos[tile + 1];  // tiles hold <= lt rows
+    if constexpr (MX) {
+        // the first index and weight batches are requested before the staging, which does not
+        // need them, so their latency overlaps the record loads (277 -> 271 us)
+        LdsMfmaGather<NOPS, UB, PD> g;
+        if (live) g.start(a.loc_idx, (const float*)a.w_t, p, a.width, t - rl * SG);
+        ell_stage_tile<UT>((const UT*)a.xin, a.F, a.loc_nodes + k0, a.loc_off[tile + 1] - k0, lds);
+        __syncthreads();
+        if (!live) return;  // after the only barrier
+        const int row = ell_row_at(a.order, p);
+        const int fofs = (t - rl * SG) * FPT;
+        AT d[NOPS][FPT];
+        g.finish(lds, fofs, d);
+        stage_update<UT, AT, NOPS, FPT>(a, row, fofs, d);
+    } else {
+        ell_stage_tile<UT>((const UT*)a.xin, a.F, a.loc_nodes + k0, a.loc_off[tile + 1] - k0, lds);
+        __syncthreads();
+        if (!live) return;  // after the only barrier
+        const int row = ell_row_at(a.order, p);
+        const int fofs = (t - rl * SG) * FPT;
+        AT d[NOPS][FPT];
         ell_gather_dot_lds<WT, UT, AT, NOPS, FPT, UB, SG, PD>(a.loc_idx, (const WT*)a.w_t, lds, p, a.width, fofs,
                                                               t - rl * SG, d);
-    stage_update<UT, AT, NOPS, FPT>(a, row, fofs, d);
+        stage_update<UT, AT, NOPS, FPT>(a, row, fofs, d);
+    }
 }
 
 // The state streams (u, acc) are read and written once per stage: their loads and stores
