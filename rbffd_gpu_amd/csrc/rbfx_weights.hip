@@ -1389,22 +1389,20 @@ __device__ __attribute__((noinline)) void lu_team(const WParams& P, int start, i
         }
         team_sync<W>();  // multipliers, pivots and pos of this panel are visible
         const int kdone = kp + npn;  // columns < kdone are finished
-        // 1. the pivot rows' values on this wave's live tiles
+        // 1. the pivot rows' values on this wave's live tiles: every lane checks whether its
+        //    row of each row tile became the pivot of one of the panel's columns (pos), so the
+        //    rows are found without a branch per pivot
 #pragma unroll
-        for (int q = 0; q < KB; q++) {
-            if (q >= npn) break;
-            const int p = uni((int)perm[kp + q]);
-            if (rl == (p & 15)) {
+        for (int rt = 0; rt < RT; rt++) {
+            if (rt >= RTL) break;  // padding rows only (uniform)
+            const int qq = (int)pos[16 * rt + rl] - kp;
+            if (qq >= 0 && qq < npn) {
 #pragma unroll
                 for (int sl = 0; sl < NCT; sl++) {
                     const int ct = wt + W * sl;
                     if (16 * ct + 16 <= kdone || 16 * ct >= m + T) continue;
 #pragma unroll
-                    for (int rt = 0; rt < RT; rt++)
-                        if (rt == (p >> 4)) {
-#pragma unroll
-                            for (int v = 0; v < 4; v++) U12[q * US + 16 * ct + 4 * v + cg] = acc[rt][sl][v];
-                        }
+                    for (int v = 0; v < 4; v++) U12[qq * US + 16 * ct + 4 * v + cg] = acc[rt][sl][v];
                 }
             }
         }
@@ -1416,14 +1414,19 @@ __device__ __attribute__((noinline)) void lu_team(const WParams& P, int start, i
             const int c = 16 * ct + (lane & 15);
             if (16 * ct < m + T) {
                 double u[KB];
+                int pq[KB];
 #pragma unroll
-                for (int q = 0; q < KB; q++) u[q] = (q < npn) ? U12[q * US + c] : 0.0;
+                for (int q = 0; q < KB; q++) {
+                    u[q] = (q < npn) ? U12[q * US + c] : 0.0;
+                    pq[q] = (q < npn) ? (int)perm[kp + q] : 0;
+                }
+#pragma unroll
+                for (int q = 1; q < KB; q++) pq[q] = uni(pq[q]);  // one wait for all the pivot rows
 #pragma unroll
                 for (int q = 1; q < KB; q++)
                     if (q < npn) {
-                        const int pq = uni((int)perm[kp + q]);
 #pragma unroll
-                        for (int jj = 0; jj < q; jj++) u[q] = u[q] - Lb[jj * PS + pq] * u[jj];
+                        for (int jj = 0; jj < q; jj++) u[q] = u[q] - Lb[jj * PS + pq[q]] * u[jj];
                     }
                 const bool fin = c < kdone;
 #pragma unroll
