@@ -319,7 +319,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-steps", type=int, default=1,
                     help="CPU baseline RK4 steps per timed run (median of 5 runs after one warm-up run)")
-    ap.add_argument("--kernel-only", action="store_true", help="set up, then launch only stage kernels (profiling)")
+    ap.add_argument("--kernel-only", action="store_true",
+                    help="set up, then launch only the stage kernels, the SpMV and the fused SpMM (profiling)")
     ap.add_argument("--no-weights", action="store_true", help="skip the batched uneven-stencil weights section")
     ap.add_argument("--no-revolution", action="store_true", help="skip the untimed one-revolution accuracy run")
     ap.add_argument("--weights-eps-scale", type=float, default=1.0,
@@ -492,6 +493,16 @@ def main():
         d_y = ctx.alloc(N, np.float64)
         for _ in range(20):
             ell_sp64.spmv(0, d_x, d_y)
+        # the fused SpMM (4 operators x F fields, every output written), tuned, then 20 launches
+        for ell, dt_np, ft in ((ell64, np.float64, R.F64), (ell32, np.float32, R.F32)):
+            d_xs = ctx.to_device(u0.astype(dt_np))
+            d_ys = ctx.alloc((4, N, F), dt_np)
+            TU.tune_spmm(ctx, ell, F, d_xs, d_ys, ft, ft, table=table)
+            for _ in range(20):
+                ell.spmm(F, d_xs, d_ys, ft, ft)
+            ctx.sync()
+            d_xs.free()
+            d_ys.free()
         stream_copy_gbs(ctx, R, reps=20)
         ctx.sync()
         log("kernel-only launches done")
