@@ -119,10 +119,10 @@ __global__ void __launch_bounds__(256) ell_spmm_kernel(const int32_t* __restrict
 // (rbfx_ell_build_local). The tile's distinct neighbour records are copied into LDS once, then
 // every row gathers from there through 16-bit local indices; the summation order is that of
 // ell_spmm_kernel, so the outputs are the same bit for bit. MX: FP32 multiply-adds on the
-// matrix cores (16 fields on 4 lanes per row). FP32-accumulation instances: six waves per SIMD
-// (three 512-thread blocks per CU), as in rk4_stage_lds_kernel.
+// matrix cores (16 fields on 4 lanes per row). FP32-accumulation instances with 4 fields per
+// thread: six waves per SIMD (three 512-thread blocks per CU), as in rk4_stage_lds_kernel.
 template <typename WT, typename UT, typename AT, int NOPS, int FPT, int UB, int SG, int PD, bool MX>
-__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu((sizeof(UT) == 4 && sizeof(AT) == 4) ? 6 : 1))) ell_spmm_lds_kernel(const int32_t* __restrict__ loc_pos,
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu((sizeof(UT) == 4 && sizeof(AT) == 4 && FPT == 4) ? 6 : 1))) ell_spmm_lds_kernel(const int32_t* __restrict__ loc_pos,
                                                            const int32_t* __restrict__ loc_off,
                                                            const int32_t* __restrict__ loc_nodes,
                                                            const uint16_t* __restrict__ loc_idx,

@@ -94,11 +94,12 @@ __global__ void __launch_bounds__(256) rk4_stage_kernel(StageArgs a) {
 // tiles with 4 lanes per row); the tile's distinct neighbour records are copied into LDS
 // first, then every row gathers from there.
 // MX: FP32 multiply-adds on the matrix cores (16 fields, 4 lanes per row).
-// The FP32-accumulation instances are compiled for six waves per SIMD (at most 80 VGPRs):
+// The FP32-accumulation instances with 4 fields per thread (the 128-row tiles' shapes) are
+// compiled for six waves per SIMD (at most 80 VGPRs):
 // three 512-thread blocks per CU, as many as 48 KB of staged records allow; at the 92 VGPRs the
 // compiler chose unconstrained, only two fitted (FP32 stage at N = 1M, n = 50: 271 -> 266 us).
 template <typename WT, typename UT, typename AT, int NOPS, int FPT, int UB, int SG, int PD, bool MX = false>
-__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu((sizeof(UT) == 4 && sizeof(AT) == 4) ? 6 : 1))) rk4_stage_lds_kernel(StageArgs a) {
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu((sizeof(UT) == 4 && sizeof(AT) == 4 && FPT == 4) ? 6 : 1))) rk4_stage_lds_kernel(StageArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     UT* lds = reinterpret_cast<UT*>(smem);
     const int tile = xcd_tile(blockIdx.x, a.nb);
