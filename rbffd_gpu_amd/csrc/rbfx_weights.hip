@@ -1082,7 +1082,7 @@ typedef double d4v __attribute__((ext_vector_type(4)));
 
 constexpr int LU_BLOCK = 256;   // threads per workgroup (4 waves)
 constexpr int LU_WAVES = LU_BLOCK / WAVE;
-constexpr int LU_NLAYOUT = 7;   // layout 6 (three column tiles per wave) only in the wide kernel
+constexpr int LU_NLAYOUT = 7;   // layouts 5 and 6 (128 and 192 doubles of tiles per lane) only in the wide kernel
 
 struct LuLayout {
     int W, RT, NCT;
@@ -1642,8 +1642,10 @@ __device__ __attribute__((noinline)) void lu_team(const WParams& P, int start, i
 }
 
 // Work unit: {first position in P.order, stencils (<= teams of the layout), layout | n << 8, m}.
-// The narrow kernel (layouts 0-5, tiles of at most 64 doubles per lane) is compiled for three
-// waves per SIMD (168 VGPRs); the wide one (96 doubles of tiles per lane) for two.
+// The narrow kernel (layouts 0-4, tiles of at most 56 doubles per lane) is compiled for three
+// waves per SIMD (168 VGPRs); the wide one (layouts 5 and 6: 64 and 96 doubles of tiles per
+// lane) for two. With 64 doubles of tiles in 168 VGPRs the (4, 8, 2) layout spilled them; in the
+// wide kernel the degree-3 set (m up to 121) takes 0.200 s instead of 0.209 s.
 template <bool WIDE>
 __global__ void __launch_bounds__(LU_BLOCK) __attribute__((amdgpu_waves_per_eu(WIDE ? 2 : 3))) weights_lu_kernel(WParams P, const int4* __restrict__ units, int n_units,
                                                              int* __restrict__ counter) {
@@ -1663,7 +1665,9 @@ __global__ void __launch_bounds__(LU_BLOCK) __attribute__((amdgpu_waves_per_eu(W
             case 2: lu_team<2, 4, 2, OCC>(P, U.x, U.y, n, m, smem); break;
             case 3: lu_team<4, 6, 2, OCC>(P, U.x, U.y, n, m, smem); break;
             case 4: lu_team<4, 7, 2, OCC>(P, U.x, U.y, n, m, smem); break;
-            case 5: lu_team<4, 8, 2, OCC>(P, U.x, U.y, n, m, smem); break;
+            case 5:
+                if constexpr (WIDE) lu_team<4, 8, 2, OCC>(P, U.x, U.y, n, m, smem);
+                break;
             default:
                 if constexpr (WIDE) lu_team<4, 8, 3, OCC>(P, U.x, U.y, n, m, smem);
                 break;
@@ -1823,7 +1827,7 @@ int weights_impl(rbfx_ctx* ctx, int32_t n_stencils, const int32_t* d_st_off, con
             const int id = lu_pick(m, T);
             if (id < 0) return rbfx_fail(ctx, RBFX_E_TOO_LARGE, "weights: system %d x %d with %d RHS has no layout", m, m, T);
             const LuLayout L = lu_layout(id);
-            wide |= id >= 6;
+            wide |= id >= 5;
             const int teams = LU_WAVES / L.W;
             lds = std::max(lds, (size_t)teams * lu_team_bytes(L.W, L.RT, L.NCT, m, n, T));
             for (int i = 0; i < c; i += teams)
