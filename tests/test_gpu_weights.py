@@ -270,3 +270,54 @@ def test_weights_f32_ir_uneven_mix(ctx):
             ref = w_ref[:, a:b]
             rel = np.abs(w32[:, a:b] - ref).max() / np.abs(ref).max()
             assert rel <= min(1.0, 64 * kappa * U64), (deg, s, rel, kappa)
+
+
+def test_weights_uneven_full_size_properties(ctx):
+    """The uneven-stencil workload at full size (10^6 stencils, sizes uniform over
+    {17, 31, 50, 75, 101}, X/Y/Z/LAPL, degree 2 and 3, the production FP64 solve), checked
+    through properties that do not need a CPU solve of every system: every stencil meets the
+    backward-error bar, and the X and Laplacian weights reproduce the derivatives of a
+    polynomial of the augmentation degree (quadratic; cubic where the stencil has room for the
+    cubic monomials) to 1e-9 of the magnitude of the terms summed."""
+    print("case: uneven workload at full size, 10^6 stencils", flush=True)
+    N = 1000000
+    ks = np.array([17, 31, 50, 75, 101])
+    xyz = R.nodes_fibonacci(N, 1e-3, 0)
+    knn = R.knn_kdtree(xyz, 101)
+    sizes = np.random.default_rng(1).choice(ks, size=N)
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int32)
+    idx = knn[np.arange(101)[None, :] < sizes[:, None]].astype(np.int32)
+    del knn
+    c = {17: (0.026, 0.08), 31: (0.035, 0.1), 50: (0.044, 0.14), 75: (0.051, 0.15), 101: (0.058, 0.16)}
+    eps = np.array([c[k][0] * np.sqrt(N) - c[k][1] for k in ks])[np.searchsorted(ks, sizes)]
+    x, y, z = xyz[:, 0], xyz[:, 1], xyz[:, 2]
+    xc, yc, zc = x, y, z  # stencil s is centred on node s (row 0 of its kd-tree list)
+    assert np.array_equal(idx[off[:-1]], np.arange(N))
+    quad = 1 + 2 * x - y + 0.5 * z + x * x - x * y + 0.3 * z * z
+    quad_dx, quad_lap = 2 + 2 * xc - yc, np.full(N, 2.6)
+    cub = quad + x ** 3 + 0.5 * x * y * z - y * y * z
+    cub_dx, cub_lap = quad_dx + 3 * xc * xc + 0.5 * yc * zc, quad_lap + 6 * xc - 2 * zc
+    starts = off[:-1]
+    mask = R.X | R.Y | R.Z | R.LAPL  # columns X, Y, Z, LAPL
+    for deg in (2, 3):
+        w, dg = ctx.weights(off, idx, xyz, eps, mask, dim=3, poly_deg=deg, diag=True)
+        mmax = 101 + (10 if deg == 2 else 20)
+        assert (dg[:, 3].astype(int) == R.lib.PATH_F64).all()
+        print(f"  deg {deg}: backward error max {dg[:, 0].max():.2e}", flush=True)
+        assert dg[:, 0].max() <= 64 * mmax * U64
+        # degree 3 needs 20 monomials: 17-node stencils keep degree 2
+        use_cub = (sizes >= 20) if deg == 3 else np.zeros(N, bool)
+        for col, name in ((0, "x"), (3, "lap")):
+            wc = w[col]
+            for vals, exact_all, sel in ((quad, quad_dx if name == "x" else quad_lap, ~use_cub),
+                                         (cub, cub_dx if name == "x" else cub_lap, use_cub)):
+                if not sel.any():
+                    continue
+                pj = vals[idx]
+                got = np.add.reduceat(wc * pj, starts)
+                scale = np.add.reduceat(np.abs(wc * pj), starts) + np.abs(exact_all)
+                err = (np.abs(got - exact_all) / scale)[sel]
+                print(f"  deg {deg} {name}: {'cubic' if vals is cub else 'quadratic'} reproduction "
+                      f"max {err.max():.2e} over {sel.sum()} stencils", flush=True)
+                assert err.max() <= 1e-9, (deg, name, float(err.max()))
+        del w, dg
