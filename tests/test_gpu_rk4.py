@@ -273,3 +273,61 @@ def test_rk4_row_order_bit_identical(ctx, c1):
         rk.close()
         ell.close()
     assert np.array_equal(outs[0], outs[1])
+
+
+def test_rk4_full_size_matches_oracle(ctx):
+    """The RK4 workload at full size (N = 10^6 Morton-numbered sphere nodes, n = 50 kd-tree
+    stencils, surface d/dx, d/dy, d/dz and Delta^4 hyperviscosity x 16 fields, the production
+    LDS tiles): two FP64 steps agree with the oracle's FP64 steps on the same weights to 1e-12;
+    the FP32 matrix-core stage equals the FP32 VALU stage bit for bit, and deviates from FP64 by
+    less than 1e-6 after the two steps."""
+    print("case: rk4 at full size, N = 10^6, n = 50, 4 operators x 16 fields", flush=True)
+    from rbffd_gpu_amd import tune as TU
+    N, n, F = 1000000, 50, 16
+    xyz = R.nodes_fibonacci(N, 1e-3, 0)
+    xyz /= np.linalg.norm(xyz, axis=1)[:, None]
+    xyz = np.ascontiguousarray(xyz[R.order_morton(xyz)])
+    st = R.knn_kdtree(xyz, n)
+    mask = R.XSFC | R.YSFC | R.ZSFC | R.HV  # columns HV, XSFC, YSFC, ZSFC (ascending bits)
+    off, idx = R.csr_from_fixed(st)
+    w, _ = ctx.weights(off, idx, xyz, 4.0 * (0.044 * np.sqrt(N) - 0.14), mask, dim=2, poly_deg=2 | R.POLY_SPHERE,
+                       hv_k=4, hv_scale=-2.0 ** 9 * float(N) ** -4)
+    ws = [w[t].reshape(N, n) for t in range(4)]
+    del w
+    axis = np.array([0.3, -0.2, 1.0])
+    vel = np.cross(2 * np.pi * axis / np.linalg.norm(axis), xyz)
+    coefs = [None, -vel[:, 0], -vel[:, 1], -vel[:, 2]]
+    u0 = _fields(xyz, F, seed=5)
+    dt = 2.5e-4
+    ref = O.rk4_step(st, ws, coefs, u0, dt, n_steps=2)
+    order = R.order_morton(xyz)
+    ell64 = R.EllOperators(ctx, st, ws, storage=R.F64)
+    ell64.set_row_order(order)
+    ell64.build_local(64, 36 * 1024 // (F * 8))
+    rk64 = R.RK4(ctx, ell64, F, R.F64, R.F64, coefs=coefs)
+    rk64.set_state(u0)
+    rk64.step(dt, 2)
+    got64 = rk64.get_state()
+    err = np.abs(got64 - ref).max() / np.abs(ref).max()
+    print(f"  FP64 vs oracle: {err:.2e}", flush=True)
+    assert err <= 1e-12
+    R.RBFX.rbfx_tune_clear()
+    ell32 = R.EllOperators(ctx, st, ws, storage=R.F32)
+    ell32.set_row_order(order)
+    tl, k_max, n_tiles = ell32.build_local(128, 48 * 1024 // (F * 4))
+    assert ell32.tune_get(TU.KIND_RK4, F, R.F32, R.F32) == (4, tl * 4, 4, 5)  # the matrix-core shape
+    rk32 = R.RK4(ctx, ell32, F, R.F32, R.F32, coefs=coefs)
+    rk32.set_state(u0)
+    rk32.step(dt, 2)
+    mx = rk32.get_state()
+    ell32.tune_set(TU.KIND_RK4, F, R.F32, R.F32, 4, tl * 4, 4, 4)  # the VALU LDS-staged shape
+    rk32.set_state(u0)
+    rk32.step(dt, 2)
+    assert np.array_equal(mx, rk32.get_state())
+    dev = np.abs(mx - got64).max() / np.abs(got64).max()
+    print(f"  FP32 (matrix cores) vs FP64 after 2 steps: {dev:.2e}; {n_tiles} tiles of <= {tl} rows, "
+          f"<= {k_max} nodes", flush=True)
+    assert dev <= 1e-6
+    R.RBFX.rbfx_tune_clear()
+    for o in (rk32, ell32, rk64, ell64):
+        o.close()
