@@ -168,7 +168,7 @@ int rbfx_weights(rbfx_ctx* ctx, int32_t n_stencils, const int32_t* d_st_off, con
 /* rbfx_weights that allocates no device memory. No counterpart in the reference: it is for
  * callers that cannot allocate (code under stream capture, serving loops, harnesses that
  * forbid it). Same arguments, same kernels, same results. The solver's scratch (stencil order,
- * per-stencil flags, work units, work counter: about 24 bytes per stencil) lives in a
+ * per-stencil flags, size-class table, work counter: about 8 bytes per stencil) lives in a
  * file-scope device array sized for RBFX_WEIGHTS_NOALLOC_MAX_STENCILS stencils, which each
  * device holds until the process ends; n_stencils above it is RBFX_E_TOO_LARGE. Every call on
  * a device shares that device's array, so calls that overlap on one device (several contexts

@@ -1641,35 +1641,43 @@ __device__ __attribute__((noinline)) void lu_team(const WParams& P, int start, i
     }
 }
 
-// Work unit: {first position in P.order, stencils (<= teams of the layout), layout | n << 8, m}.
 // The narrow kernel (layouts 0-4, tiles of at most 56 doubles per lane) is compiled for three
 // waves per SIMD (168 VGPRs); the wide one (layouts 5 and 6: 64 and 96 doubles of tiles per
 // lane) for two. With 64 doubles of tiles in 168 VGPRs the (4, 8, 2) layout spilled them; in the
 // wide kernel the degree-3 set (m up to 121) takes 0.200 s instead of 0.209 s.
+// Work units are numbered class by class (largest systems first): class c holds the units
+// ustart[c] .. ustart[c + 1] - 1, and its record classes[c] = {first position in P.order,
+// stencils, layout | n << 8, m}; unit k of the class solves the class's stencils k * teams ..
+// k * teams + teams - 1 (teams = workgroup waves / layout waves).
 template <bool WIDE>
-__global__ void __launch_bounds__(LU_BLOCK) __attribute__((amdgpu_waves_per_eu(WIDE ? 2 : 3))) weights_lu_kernel(WParams P, const int4* __restrict__ units, int n_units,
-                                                             int* __restrict__ counter) {
+__global__ void __launch_bounds__(LU_BLOCK) __attribute__((amdgpu_waves_per_eu(WIDE ? 2 : 3))) weights_lu_kernel(WParams P, const int4* __restrict__ classes, const int* __restrict__ ustart,
+                                                             int n_cls, int n_units, int* __restrict__ counter) {
     constexpr int OCC = WIDE ? 2 : 3;
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ int s_unit;
     for (;;) {
         if (threadIdx.x == 0) s_unit = atomicAdd(counter, 1);
         __syncthreads();
-        const int u = s_unit;
+        const int u = __builtin_amdgcn_readfirstlane(s_unit);
         if (u >= n_units) return;  // the same value in every thread
-        const int4 U = units[u];
-        const int lay = U.z & 0xff, n = U.z >> 8, m = U.w;
+        int c = 0;
+        while (c + 1 < n_cls && u >= ustart[c + 1]) c++;
+        const int4 C = classes[c];
+        const int lay = C.z & 0xff, n = C.z >> 8, m = C.w;
+        const int teams = LU_WAVES / lu_layout(lay).W;
+        const int k = (u - ustart[c]) * teams;
+        const int first = C.x + k, count = min(teams, C.y - k);
         switch (lay) {
-            case 0: lu_team<1, 2, 2, OCC>(P, U.x, U.y, n, m, smem); break;
-            case 1: lu_team<2, 3, 2, OCC>(P, U.x, U.y, n, m, smem); break;
-            case 2: lu_team<2, 4, 2, OCC>(P, U.x, U.y, n, m, smem); break;
-            case 3: lu_team<4, 6, 2, OCC>(P, U.x, U.y, n, m, smem); break;
-            case 4: lu_team<4, 7, 2, OCC>(P, U.x, U.y, n, m, smem); break;
+            case 0: lu_team<1, 2, 2, OCC>(P, first, count, n, m, smem); break;
+            case 1: lu_team<2, 3, 2, OCC>(P, first, count, n, m, smem); break;
+            case 2: lu_team<2, 4, 2, OCC>(P, first, count, n, m, smem); break;
+            case 3: lu_team<4, 6, 2, OCC>(P, first, count, n, m, smem); break;
+            case 4: lu_team<4, 7, 2, OCC>(P, first, count, n, m, smem); break;
             case 5:
-                if constexpr (WIDE) lu_team<4, 8, 2, OCC>(P, U.x, U.y, n, m, smem);
+                if constexpr (WIDE) lu_team<4, 8, 2, OCC>(P, first, count, n, m, smem);
                 break;
             default:
-                if constexpr (WIDE) lu_team<4, 8, 3, OCC>(P, U.x, U.y, n, m, smem);
+                if constexpr (WIDE) lu_team<4, 8, 3, OCC>(P, first, count, n, m, smem);
                 break;
         }
         __syncthreads();  // the team LDS and s_unit are free for the next unit
@@ -1709,29 +1717,31 @@ F64Solver f64_solver() {
     return F64Solver::LU;
 }
 
-// Device scratch of one call: [order: n][flags: n + 2][units: n int4][counter]. 256-byte
-// aligned pieces.
+// Device scratch of one call: [order: n][flags: n + 2][classes: MAX_M + 1 int4][ustart:
+// MAX_M + 2][counter]. 256-byte aligned pieces.
 struct WScratch {
     int32_t* order;
     int* flags;
-    int4* units;
+    int4* classes;
+    int* ustart;
     int* counter;
 };
 constexpr size_t wscratch_al(size_t b) { return (b + 255) / 256 * 256; }
 constexpr size_t wscratch_bytes(int n) {
     return wscratch_al(sizeof(int32_t) * (size_t)n) + wscratch_al(sizeof(int) * ((size_t)n + 2)) +
-           wscratch_al(sizeof(int4) * (size_t)n) + 256;
+           wscratch_al(sizeof(int4) * (MAX_M + 1)) + wscratch_al(sizeof(int) * (MAX_M + 2)) + 256;
 }
 WScratch wscratch_carve(void* base, int n) {
-    auto al = [](size_t b) { return (b + 255) / 256 * 256; };
     unsigned char* p = (unsigned char*)base;
     WScratch w;
     w.order = (int32_t*)p;
-    p += al(sizeof(int32_t) * (size_t)n);
+    p += wscratch_al(sizeof(int32_t) * (size_t)n);
     w.flags = (int*)p;
-    p += al(sizeof(int) * ((size_t)n + 2));
-    w.units = (int4*)p;
-    p += al(sizeof(int4) * (size_t)n);
+    p += wscratch_al(sizeof(int) * ((size_t)n + 2));
+    w.classes = (int4*)p;
+    p += wscratch_al(sizeof(int4) * (MAX_M + 1));
+    w.ustart = (int*)p;
+    p += wscratch_al(sizeof(int) * (MAX_M + 2));
     w.counter = (int*)p;
     return w;
 }
@@ -1814,10 +1824,14 @@ int weights_impl(rbfx_ctx* ctx, int32_t n_stencils, const int32_t* d_st_off, con
     const F64Solver solver = f64_solver();
 
     // One persistent launch of the matrix-core LU over the classes sizes_count / sizes_start of
-    // the order array (largest systems first). The units go to the scratch's unit table.
-    std::vector<int4> units;
+    // the order array (largest systems first). The class table (per class its stencils and
+    // layout, and the number of the class's first work unit) goes to the scratch; the kernel
+    // derives each unit from it.
+    std::vector<int4> classes;
+    std::vector<int> ustart;
     auto launch_persistent = [&](const std::vector<int>& sizes_count, const std::vector<int>& sizes_start) -> int {
-        units.clear();
+        classes.clear();
+        ustart.assign(1, 0);
         size_t lds = 0;
         bool wide = false;
         for (int n = MAX_M; n >= 1; n--) {
@@ -1830,14 +1844,16 @@ int weights_impl(rbfx_ctx* ctx, int32_t n_stencils, const int32_t* d_st_off, con
             wide |= id >= 5;
             const int teams = LU_WAVES / L.W;
             lds = std::max(lds, (size_t)teams * lu_team_bytes(L.W, L.RT, L.NCT, m, n, T));
-            for (int i = 0; i < c; i += teams)
-                units.push_back(make_int4(sizes_start[n] + i, std::min(teams, c - i), id | (n << 8), m));
+            classes.push_back(make_int4(sizes_start[n], c, id | (n << 8), m));
+            ustart.push_back(ustart.back() + (c + teams - 1) / teams);
         }
-        if (units.empty()) return RBFX_OK;
+        const int n_units = ustart.back();
+        if (n_units == 0) return RBFX_OK;
         if (lds + STATIC_LDS_RESERVE > lds_limit)
             return rbfx_fail(ctx, RBFX_E_TOO_LARGE, "weights: matrix-core LU needs %zu B of LDS", lds);
         hipError_t le;
-        if ((le = hipMemcpyAsync(W.units, units.data(), sizeof(int4) * units.size(), hipMemcpyHostToDevice, st)) != hipSuccess)
+        if ((le = hipMemcpyAsync(W.classes, classes.data(), sizeof(int4) * classes.size(), hipMemcpyHostToDevice, st)) != hipSuccess ||
+            (le = hipMemcpyAsync(W.ustart, ustart.data(), sizeof(int) * ustart.size(), hipMemcpyHostToDevice, st)) != hipSuccess)
             return rbfx_fail(ctx, RBFX_E_HIP, "weights: %s", hipGetErrorString(le));
         hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(64), 0, st, W.counter, 1);
         int per_cu = 0;
@@ -1845,15 +1861,16 @@ int weights_impl(rbfx_ctx* ctx, int32_t n_stencils, const int32_t* d_st_off, con
                   : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, weights_lu_kernel<false>, LU_BLOCK, lds);
         if (le != hipSuccess || per_cu < 1)
             return rbfx_fail(ctx, RBFX_E_HIP, "weights: no occupancy for the matrix-core LU (%zu B LDS)", lds);
-        const int grid = (int)std::min<long long>((long long)units.size(), (long long)per_cu * std::max(ctx->n_cu, 1));
+        const int grid = (int)std::min<long long>((long long)n_units, (long long)per_cu * std::max(ctx->n_cu, 1));
         WParams Q = P;
         Q.fp32 = 0;
+        const int n_cls = (int)classes.size();
         if (wide)
-            hipLaunchKernelGGL(weights_lu_kernel<true>, dim3(grid), dim3(LU_BLOCK), lds, st, Q, (const int4*)W.units,
-                               (int)units.size(), W.counter);
+            hipLaunchKernelGGL(weights_lu_kernel<true>, dim3(grid), dim3(LU_BLOCK), lds, st, Q, (const int4*)W.classes,
+                               (const int*)W.ustart, n_cls, n_units, W.counter);
         else
-            hipLaunchKernelGGL(weights_lu_kernel<false>, dim3(grid), dim3(LU_BLOCK), lds, st, Q, (const int4*)W.units,
-                               (int)units.size(), W.counter);
+            hipLaunchKernelGGL(weights_lu_kernel<false>, dim3(grid), dim3(LU_BLOCK), lds, st, Q, (const int4*)W.classes,
+                               (const int*)W.ustart, n_cls, n_units, W.counter);
         le = hipGetLastError();
         if (le != hipSuccess) return rbfx_fail(ctx, RBFX_E_HIP, "weights: matrix-core LU launch failed: %s", hipGetErrorString(le));
         return RBFX_OK;
