@@ -81,6 +81,24 @@ void orc_sol_norms(int32_t n, const double* approx, const double* exact, double*
 /* FP32 weight storage keeping the FP64 row sums (this build's rounding; row-major in/out). */
 void orc_round_f32_rowsum(int32_t n_rows, int32_t width, const double* w, float* out);
 
+/* Steady Stokes (oracle_stokes.cpp). Matrices are CSR with int64 row offsets and ascending
+ * columns. w holds the LSFC, XSFC, YSFC, ZSFC weights of fixed-width stencils (4 x N*n,
+ * row-major, idx[i*n + j]). */
+int64_t orc_stokes_nnz(int32_t N, int32_t n);
+void orc_stokes_assemble(int32_t N, int32_t n, const int32_t* idx, const double* w, double eta, int64_t* ip,
+                         int32_t* ind, double* val);
+void orc_csr_matvec(int32_t nrows, const int64_t* ip, const int32_t* ind, const double* val, const double* x,
+                    double* y);
+/* ILU(0) restricted to rows/columns [row_start, row_end); returns the factor's nnz
+ * (lip/lind/lval sized like the input). */
+int64_t orc_ilu0(int32_t nrows, const int64_t* ip, const int32_t* ind, const double* val, int32_t row_start,
+                 int32_t row_end, int64_t* lip, int32_t* lind, double* lval);
+void orc_ilu0_apply(int32_t nrows, const int64_t* lip, const int32_t* lind, const double* lval, double* v);
+/* GMRES(krylov) with left ILU(0) preconditioning (lip NULL: none). Returns 0 when converged. */
+int orc_gmres(int32_t nrows, const int64_t* ip, const int32_t* ind, const double* val, const int64_t* lip,
+              const int32_t* lind, const double* lval, const double* b, double tol, int32_t max_iter,
+              int32_t krylov, double* x, int32_t* iters, double* error);
+
 #ifdef __cplusplus
 }
 #endif

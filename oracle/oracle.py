@@ -82,6 +82,18 @@ def lib():
         L.orc_sol_norms.argtypes = [i32, vp, vp, vp]
         L.orc_round_f32_rowsum.restype = None
         L.orc_round_f32_rowsum.argtypes = [i32, i32, vp, vp]
+        L.orc_stokes_nnz.restype = C.c_int64
+        L.orc_stokes_nnz.argtypes = [i32, i32]
+        L.orc_stokes_assemble.restype = None
+        L.orc_stokes_assemble.argtypes = [i32, i32, vp, vp, d, vp, vp, vp]
+        L.orc_csr_matvec.restype = None
+        L.orc_csr_matvec.argtypes = [i32, vp, vp, vp, vp, vp]
+        L.orc_ilu0.restype = C.c_int64
+        L.orc_ilu0.argtypes = [i32, vp, vp, vp, i32, i32, vp, vp, vp]
+        L.orc_ilu0_apply.restype = None
+        L.orc_ilu0_apply.argtypes = [i32, vp, vp, vp, vp]
+        L.orc_gmres.restype = C.c_int
+        L.orc_gmres.argtypes = [i32, vp, vp, vp, vp, vp, vp, vp, d, i32, i32, vp, C.POINTER(i32), C.POINTER(d)]
         _lib = L
     return _lib
 
@@ -266,3 +278,68 @@ def knn_brute(xyz, k):
         d = ((xyz - xyz[i]) ** 2).sum(1)
         out[i] = np.lexsort((np.arange(d.size), d))[:k]
     return out
+
+
+# ---- steady Stokes (oracle_stokes.cpp) ----------------------------------------------------
+
+class CSR:
+    """CSR matrix with int64 row offsets (indptr), int32 columns (ascending) and FP64 values."""
+
+    def __init__(self, indptr, indices, data, n):
+        self.indptr = np.ascontiguousarray(indptr, np.int64)
+        self.indices = np.ascontiguousarray(indices, np.int32)
+        self.data = np.ascontiguousarray(data, np.float64)
+        self.n = n
+
+    def scipy(self):
+        import scipy.sparse as sp
+        return sp.csr_matrix((self.data, self.indices, self.indptr), shape=(self.n, self.n))
+
+    def matvec(self, x):
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.empty(self.n)
+        lib().orc_csr_matvec(self.n, _p(self.indptr), _p(self.indices), _p(self.data), _p(x), _p(y))
+        return y
+
+
+def stokes_assemble(idx, w4, eta=1.0):
+    """Steady-Stokes matrix (stokes_steady_pde.cpp:61-230) from fixed-width stencils idx (N, n)
+    and the LSFC, XSFC, YSFC, ZSFC weights w4 (4, N, n)."""
+    idx = np.ascontiguousarray(idx, np.int32)
+    N, n = idx.shape
+    w4 = np.ascontiguousarray(w4, np.float64).reshape(4, N, n)
+    nnz = lib().orc_stokes_nnz(N, n)
+    ip = np.empty(4 * N + 5, np.int64)
+    ind = np.empty(nnz, np.int32)
+    val = np.empty(nnz)
+    lib().orc_stokes_assemble(N, n, _p(idx), _p(w4), eta, _p(ip), _p(ind), _p(val))
+    return CSR(ip, ind, val, 4 * N + 4)
+
+
+def ilu0(A, row_start=0, row_end=None):
+    """ILU(0) of the block [row_start, row_end)^2 of A, identity rows elsewhere (ilu0.hpp)."""
+    row_end = A.n if row_end is None else row_end
+    lip = np.empty(A.n + 1, np.int64)
+    lind = np.empty(A.indices.size, np.int32)
+    lval = np.empty(A.indices.size)
+    nnz = lib().orc_ilu0(A.n, _p(A.indptr), _p(A.indices), _p(A.data), row_start, row_end, _p(lip), _p(lind), _p(lval))
+    return CSR(lip, lind[:nnz].copy(), lval[:nnz].copy(), A.n)
+
+
+def ilu0_apply(F, v):
+    v = np.array(v, np.float64, copy=True)
+    lib().orc_ilu0_apply(F.n, _p(F.indptr), _p(F.indices), _p(F.data), _p(v))
+    return v
+
+
+def gmres(A, b, F=None, tol=1e-8, max_iter=300, krylov=20):
+    """ViennaCL-style GMRES(krylov) with left preconditioner F (an ilu0() factor, or None).
+    Returns (x, converged, iterations, error estimate ||M^-1 r|| / ||b||)."""
+    b = np.ascontiguousarray(b, np.float64)
+    x = np.empty(A.n)
+    it, err = C.c_int32(0), C.c_double(0.0)
+    z = None
+    fp = (_p(F.indptr), _p(F.indices), _p(F.data)) if F is not None else (z, z, z)
+    rc = lib().orc_gmres(A.n, _p(A.indptr), _p(A.indices), _p(A.data), *fp, _p(b), tol, max_iter, krylov, _p(x),
+                         C.byref(it), C.byref(err))
+    return x, rc == 0, it.value, err.value
