@@ -426,9 +426,9 @@ def main():
         log(f"Morton row order: {time.time() - t0:.2f}s")
         t0 = time.time()
         # tiles and node caps: FP32 128-row tiles with 48 KB of staged records per block (three
-        # 512-thread blocks per CU), FP64 64-row tiles with 36 KB; the fastest of sweeps over
-        # 32-, 64- and 128-row tiles and 16-56 KB caps
-        for name, e, fb, rows, cap in (("fp64", ell64, 8, 64, 36 * 1024), ("fp32", ell32, 4, 128, 48 * 1024)):
+        # 512-thread blocks per CU), FP64 128-row tiles with 78 KB (two per CU); the fastest of
+        # sweeps over 32-, 64- and 128-row tiles and 16-78 KB caps
+        for name, e, fb, rows, cap in (("fp64", ell64, 8, 128, 78 * 1024), ("fp32", ell32, 4, 128, 48 * 1024)):
             tl, k_max, n_tiles = e.build_local(rows, cap // (F * fb))
             log(f"LDS staging tables {name}: {n_tiles} tiles of <= {tl} rows, at most {k_max} distinct nodes per tile")
         log(f"LDS staging tables: {time.time() - t0:.2f}s")

@@ -303,7 +303,7 @@ def test_rk4_full_size_matches_oracle(ctx):
     order = R.order_morton(xyz)
     ell64 = R.EllOperators(ctx, st, ws, storage=R.F64)
     ell64.set_row_order(order)
-    ell64.build_local(64, 36 * 1024 // (F * 8))
+    ell64.build_local(128, 78 * 1024 // (F * 8))  # the tiles the benchmark times
     rk64 = R.RK4(ctx, ell64, F, R.F64, R.F64, coefs=coefs)
     rk64.set_state(u0)
     rk64.step(dt, 2)
