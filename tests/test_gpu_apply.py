@@ -139,13 +139,15 @@ def test_spmm_tuned_shapes_agree(ctx):
     ell.close()
 
 
+@pytest.mark.parametrize("tiles", ["short", "bench"])
 @pytest.mark.parametrize("storage", [R.F64, R.F32])
-def test_spmm_lds_staged_bit_identical(ctx, c1, storage):
+def test_spmm_lds_staged_bit_identical(ctx, c1, storage, tiles):
     """The LDS-staged SpMM shapes (tile records copied into LDS once, 16-bit local indices;
     FP32: also the matrix-core multiply-adds) take the slots in the same order as the direct
     gathers, so the fused 4-operator x 16-field outputs are bit-identical; FP64 also matches the
-    CPU restatement to 1e-12 (cancellation-normalised)."""
-    print(f"case: spmm LDS-staged shapes storage={storage}", flush=True)
+    CPU restatement to 1e-12 (cancellation-normalised). Tiles: 64 rows capped at 128 nodes
+    (many short tiles), or the benchmark's 128-row tiles (48 KB of FP32 records, 78 KB of FP64)."""
+    print(f"case: spmm LDS-staged shapes storage={storage} tiles={tiles}", flush=True)
     from rbffd_gpu_amd import tune as TU
     xyz, st = c1["xyz"], c1["st"]
     off, idx = R.csr_from_fixed(st)
@@ -158,7 +160,10 @@ def test_spmm_lds_staged_bit_identical(ctx, c1, storage):
     dt = np.float64 if ft == R.F64 else np.float32
     ell = R.EllOperators(ctx, st, ws, storage=storage)
     ell.set_row_order(R.order_morton(xyz))
-    tl, k_max, n_tiles = ell.build_local(64, 128)
+    if tiles == "short":
+        tl, k_max, n_tiles = ell.build_local(64, 128)
+    else:
+        tl, k_max, n_tiles = ell.build_local(128, (78 if ft == R.F64 else 48) * 1024 // (F * np.dtype(dt).itemsize))
     d_u = ctx.to_device(u.astype(dt))
     outs = {}
     for fpt, block, batch, share in TU.candidates(F, TU.KIND_SPMM, 4, (tl, k_max, n_tiles), 8 if ft == R.F64 else 4,
