@@ -275,6 +275,45 @@ int rbfx_tune_get(rbfx_ell* ell, int32_t kind, int32_t n_fields, int32_t field_t
                   int32_t* block, int32_t* batch, int32_t* share);
 int rbfx_tune_clear(void);
 
+/* ---- steady Stokes on the sphere ------------------------------------------------------- */
+/* StokesSteadyPDE (src/pdes/elliptic/stokes_steady_pde.cpp:61-230): the coupled system in the
+ * reference's block layout, unknowns u (0..N), v (N..2N), w (2N..3N), p (3N..4N) and the four
+ * constraint multipliers (4N..4N+3):
+ *   -eta L u + D_x p + lambda_u = f_u, ..., D_x u + D_y v + D_z w + lambda_p = f_p,
+ *   sum u = g_u, ..., sum p = g_p,
+ * with L, D_x, D_y, D_z the LSFC, XSFC, YSFC, ZSFC weights. ops64 holds those four operators
+ * (types in that order) in FP64 storage on square fixed-width stencils that contain their
+ * centre; ops32 (optional) the same weights in FP32 storage, for the FP32 inner solves.
+ * Creating the handle factors the preconditioner of ilu0_precond with ilu0_tag(0, 3N)
+ * (src/precond/ilu0.hpp:157-260, tests/util/viennacl_gmres_poisson/main_stokes.cpp:83): ILU(0)
+ * of the velocity block, identity rows for the pressure and constraint rows. The operator sets
+ * must outlive the handle. */
+typedef struct rbfx_stokes rbfx_stokes;
+int rbfx_stokes_create(rbfx_ctx* ctx, rbfx_ell* ops64, rbfx_ell* ops32, double eta, rbfx_stokes** out);
+void rbfx_stokes_destroy(rbfx_stokes* s);
+/* Unknowns (4N + 4) and the number of dependency levels of the two triangular sweeps. */
+int rbfx_stokes_info(const rbfx_stokes* s, int32_t* n_unknowns, int32_t* lower_levels, int32_t* upper_levels);
+/* d_y = A d_x (device, 4N + 4 doubles each): the reference's prod(LHS, x). */
+int rbfx_stokes_apply(rbfx_stokes* s, const double* d_x, double* d_y);
+/* The ILU(0) factor of -eta L (N rows of the stencil width, columns ascending per row, L
+ * multipliers below the diagonal, U from it): the velocity rows of the reference's LU. */
+int rbfx_stokes_factor(rbfx_stokes* s, double* d_f);
+/* d_v = M^-1 d_v (ilu0_lu_substitute, src/precond/ilu0.hpp:273-327). */
+int rbfx_stokes_precond(rbfx_stokes* s, double* d_v);
+/* viennacl::linalg::solve(LHS, RHS, gmres_tag(tol, max_iter, krylov), ilu0_precond) (ViennaCL
+ * gmres.hpp, the reference's external/opencl_libraries/include/viennacl/linalg/gmres.hpp:166-336):
+ * restarted GMRES(krylov) from x = 0 with left preconditioning, stopping when the estimate of
+ * ||M^-1 (b - A x)|| / ||b|| falls below tol or after the restarts ViennaCL allows
+ * (max_iter / krylov cycles, one fewer when krylov divides max_iter). mode RBFX_F64: all in
+ * FP64. mode RBFX_F32 (needs ops32): FP64 refinement, r = b - A x and the stopping test in
+ * FP64, each correction from one FP32 GMRES(krylov) cycle (FP32 operator, factor and basis)
+ * that reduces its preconditioned residual 1e4-fold, at most max_iter inner iterations in all.
+ * iters receives the Arnoldi steps taken, error the final relative preconditioned residual
+ * (the estimate in FP64 mode, recomputed in FP64 in FP32 mode). Not converging is not an
+ * error: compare error with tol. */
+int rbfx_stokes_gmres(rbfx_stokes* s, const double* d_b, double* d_x, double tol, int32_t max_iter, int32_t krylov,
+                      int32_t mode, int32_t* iters, double* error);
+
 /* ---- cosine bell pointwise terms (src/pdes/hyperbolic/cosine_bell.cpp:23-70) ---------- */
 /* c_lambda[i] = -(u_vel/a), c_theta[i] = -(v_vel/a) at node i (device arrays), for use as the
  * LAMBDA / THETA coefficients of rbfx_rk4_desc (the HV coefficient is 1). */

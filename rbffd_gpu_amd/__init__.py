@@ -11,6 +11,7 @@ from .lib import (  # noqa: F401
     DeviceArray,
     EllOperators,
     RK4,
+    Stokes,
     load_library,
     dertype_bits,
     X, Y, Z, LAPL, R, HV, LAMBDA, THETA, LSFC, XSFC, YSFC, ZSFC, INTERP,

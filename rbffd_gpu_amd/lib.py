@@ -97,6 +97,13 @@ def load_library(path=None):
         "rbfx_tune_clear": (C.c_int, []),
         "rbfx_cosine_bell_coefs": (C.c_int, [vp, i32, vp, f64, f64, f64, vp, vp]),
         "rbfx_cosine_bell_init": (C.c_int, [vp, i32, vp, f64, vp, i32]),
+        "rbfx_stokes_create": (C.c_int, [vp, vp, vp, f64, C.POINTER(vp)]),
+        "rbfx_stokes_destroy": (None, [vp]),
+        "rbfx_stokes_info": (C.c_int, [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)]),
+        "rbfx_stokes_apply": (C.c_int, [vp, vp, vp]),
+        "rbfx_stokes_factor": (C.c_int, [vp, vp]),
+        "rbfx_stokes_precond": (C.c_int, [vp, vp]),
+        "rbfx_stokes_gmres": (C.c_int, [vp, vp, vp, f64, i32, i32, i32, C.POINTER(i32), C.POINTER(f64)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -472,4 +479,66 @@ class RK4:
     def close(self):
         if self.h:
             RBFX.rbfx_rk4_destroy(self.h)
+            self.h = None
+
+
+class Stokes:
+    """Steady Stokes system on the sphere (rbfx_stokes_*): ops64 / ops32 are EllOperators with the
+    LSFC, XSFC, YSFC, ZSFC weights in FP64 / FP32 storage; vectors are 4N + 4 doubles in the
+    block layout u, v, w, p, constraint multipliers."""
+
+    def __init__(self, ctx, ops64, ops32=None, eta=1.0):
+        self.ctx = ctx
+        self._ops = (ops64, ops32)  # the handle reads their device arrays
+        h = C.c_void_p()
+        ctx.check(RBFX.rbfx_stokes_create(ctx.h, ops64.h, ops32.h if ops32 is not None else None, eta, C.byref(h)),
+                  "stokes_create")
+        self.h = h
+        n, nl, nu = C.c_int32(), C.c_int32(), C.c_int32()
+        ctx.check(RBFX.rbfx_stokes_info(h, C.byref(n), C.byref(nl), C.byref(nu)), "stokes_info")
+        self.n, self.lower_levels, self.upper_levels = n.value, nl.value, nu.value
+        self.N = ops64.n_rows
+        self.width = ops64.width
+
+    def _dev(self, v):
+        return v if isinstance(v, (DeviceArray, DeviceView)) else self.ctx.to_device(np.asarray(v, np.float64))
+
+    def apply(self, x):
+        dx = self._dev(x)
+        dy = self.ctx.alloc(self.n, np.float64)
+        self.ctx.check(RBFX.rbfx_stokes_apply(self.h, dx.ptr, dy.ptr), "stokes_apply")
+        out = dy.download()
+        dy.free()
+        return out
+
+    def factor(self):
+        d = self.ctx.alloc((self.N, self.width), np.float64)
+        self.ctx.check(RBFX.rbfx_stokes_factor(self.h, d.ptr), "stokes_factor")
+        out = d.download()
+        d.free()
+        return out
+
+    def precond(self, v):
+        d = self.ctx.to_device(np.array(v, np.float64))
+        self.ctx.check(RBFX.rbfx_stokes_precond(self.h, d.ptr), "stokes_precond")
+        out = d.download()
+        d.free()
+        return out
+
+    def gmres(self, b, tol=1e-8, max_iter=300, krylov=50, mode=F64, device_out=False):
+        """Returns (x, iterations, error); x stays on the device with device_out."""
+        db = self._dev(b)
+        dx = self.ctx.alloc(self.n, np.float64)
+        it, err = C.c_int32(), C.c_double()
+        self.ctx.check(RBFX.rbfx_stokes_gmres(self.h, db.ptr, dx.ptr, tol, max_iter, krylov, mode, C.byref(it),
+                                              C.byref(err)), "stokes_gmres")
+        if device_out:
+            return dx, it.value, err.value
+        out = dx.download()
+        dx.free()
+        return out, it.value, err.value
+
+    def close(self):
+        if self.h:
+            RBFX.rbfx_stokes_destroy(self.h)
             self.h = None
