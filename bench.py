@@ -312,6 +312,101 @@ def bench_weights(ctx, R, N, cpu, nthreads, degrees=(2, 3), modes=("f64_lu", "f3
     return res
 
 
+def bench_stokes(ctx, R, cpu, level=6, n=31, budget=1000, krylov=50, tol=1e-8, cpu_iters=50):
+    """Steady Stokes on the sphere (the reference's StokesSteadyPDE with GMRES(50) + ILU(0) of the
+    velocity block): icosahedral nodes (level 6: N = 40962), n = 31 kd-tree stencils, eps =
+    0.035 sqrt(N) - 0.1, eta = 1, a seeded body force of spherical harmonics of degree <= 10 on
+    the velocity rows. Times the FP64 solve and the FP32-inner / FP64-refinement solve over the
+    same iteration budget, the operator and preconditioner applications, and the CPU
+    restatement's GMRES on a bounded sample of iterations."""
+    xyz = R.nodes_icosahedral(level)
+    xyz = np.ascontiguousarray(xyz[R.order_morton(xyz)])
+    N = xyz.shape[0]
+    st = R.knn_kdtree(xyz, n)
+    off, idx = R.csr_from_fixed(st)
+    eps = 0.035 * np.sqrt(N) - 0.1
+    types = [R.LSFC, R.XSFC, R.YSFC, R.ZSFC]
+    w, _ = ctx.weights(off, idx, xyz, eps, R.LSFC | R.XSFC | R.YSFC | R.ZSFC, dim=3, poly_deg=0)
+    ws = [w[t].reshape(N, n) for t in range(4)]
+    order = R.order_morton(xyz)
+    e64 = R.EllOperators(ctx, st, ws, storage=R.F64, types=types)
+    e32 = R.EllOperators(ctx, st, ws, storage=R.F32, types=types)
+    for e in (e64, e32):
+        e.set_row_order(order)
+    ctx.sync()
+    t0 = time.perf_counter()
+    S = R.Stokes(ctx, e64, e32)
+    ctx.sync()
+    t_setup = time.perf_counter() - t0
+    rng = np.random.default_rng(0)
+
+    def field():
+        f = np.zeros(N)
+        for _ in range(8):
+            l = int(rng.integers(1, 11))
+            f += rng.standard_normal() * R.sph_harm(l, int(rng.integers(-l, l + 1)), xyz)
+        return f
+
+    b = np.concatenate([field(), field(), field(), np.zeros(N), np.zeros(4)])
+    d_b = ctx.to_device(b)
+    S.gmres(d_b, tol, krylov, krylov)  # warm-up: code objects, workspace
+    res = {}
+    xs = {}
+    for name, mode in (("fp64", R.F64), ("fp32_ir", R.F32)):
+        ctx.sync()
+        t = time.perf_counter()
+        dx, it, err = S.gmres(d_b, tol, budget, krylov, mode, device_out=True)
+        ctx.sync()
+        sec = time.perf_counter() - t
+        xs[name] = dx.download()
+        dx.free()
+        r = b - S.apply(xs[name])
+        res[name] = {"s": round(sec, 3), "iterations": it, "ms_per_iteration": round(1e3 * sec / max(it, 1), 3),
+                     "error": err, "converged": bool(err < tol),
+                     "true_rel_residual": float(np.linalg.norm(r) / np.linalg.norm(b))}
+        log(f"Stokes {name}: {it} GMRES steps in {sec:.2f}s, preconditioned residual {err:.2e}, "
+            f"residual {res[name]['true_rel_residual']:.2e}")
+    res["fp32_ir"]["rel_diff_from_fp64"] = float(np.linalg.norm(xs["fp32_ir"] - xs["fp64"]) / np.linalg.norm(xs["fp64"]))
+    # operator and preconditioner applications (HIP events on the context stream)
+    d_x = ctx.to_device(xs["fp64"])
+    d_y = ctx.alloc(S.n, np.float64)
+    times = {}
+    for name, fn in (("apply_us", lambda: R.RBFX.rbfx_stokes_apply(S.h, d_x.ptr, d_y.ptr)),
+                     ("precond_us", lambda: R.RBFX.rbfx_stokes_precond(S.h, d_y.ptr))):
+        ctx.check(fn(), name)
+        ctx.timer_start()
+        for _ in range(20):
+            ctx.check(fn(), name)
+        times[name] = round(ctx.timer_stop() * 1e3 / 20, 1)
+    d_x.free()
+    d_y.free()
+    d_b.free()
+    out = {"N": N, "n": n, "unknowns": S.n, "eps": eps, "krylov": krylov, "tol": tol, "iteration_budget": budget,
+           "sweep_levels": {"lower": S.lower_levels, "upper": S.upper_levels}, "setup_s": round(t_setup, 3),
+           **res, **times}
+    if cpu:
+        O, build = cpu_oracle()
+        A = O.stokes_assemble(st, np.stack(ws))
+        F = O.ilu0(A, 0, 3 * N)
+        t = time.perf_counter()
+        _, _, it_c, err_c = O.gmres(A, b, F, tol=tol, max_iter=cpu_iters, krylov=krylov)
+        sec_c = time.perf_counter() - t
+        t = time.perf_counter()
+        O.ilu0_apply(F, b)
+        pre_c = time.perf_counter() - t
+        out["cpu"] = {"ms_per_iteration": round(1e3 * sec_c / it_c, 2), "precond_ms": round(1e3 * pre_c, 2),
+                      "cores": 1, "kind": "port",
+                      "sample": f"the first {it_c} GMRES({krylov}) steps of the same system with the C++ restatement "
+                                f"(oracle/oracle_stokes.cpp: ViennaCL's Householder GMRES, the reference's ILU(0); "
+                                f"{build}, one thread)"}
+        out["gpu_over_cpu_per_iteration"] = round(out["cpu"]["ms_per_iteration"] / res["fp64"]["ms_per_iteration"], 1)
+    S.close()
+    e64.close()
+    e32.close()
+    log(f"Stokes: {json.dumps(out)}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -335,6 +430,8 @@ def main():
                     help="with --weights-only: only the FP64 LU solve at degree 2 (one launch per size class)")
     ap.add_argument("--weights-sizes", default="17,31,50,75,101",
                     help="with --weights-only: comma-separated stencil sizes drawn uniformly (each of 17, 31, 50, 75, 101)")
+    ap.add_argument("--no-stokes", action="store_true", help="skip the steady-Stokes solver section")
+    ap.add_argument("--stokes-only", action="store_true", help="run only the steady-Stokes solver section")
     ap.add_argument("--node-order", action="store_true",
                     help="process rows in node order instead of the Morton locality order")
     ap.add_argument("--spiral-numbering", action="store_true",
@@ -349,6 +446,9 @@ def main():
     import rbffd_gpu_amd as R
 
     N, n, F = args.n_nodes, args.width, args.fields
+    if args.stokes_only:
+        bench_stokes(R.Context(0), R, not args.no_cpu)
+        return
     if args.weights_only:
         ctx = R.Context(0)
         sel = dict(degrees=(2,), modes=("f64_lu",)) if args.weights_f64_deg2 else {}
@@ -642,6 +742,7 @@ def main():
         for a in (ell64, ell32, ell_sp64, ell_sp32):  # the RK4 measurements are done: free HBM
             a.close()
         weights_c3 = bench_weights(ctx, R, N, not args.no_cpu, host_threads)
+    stokes = None if args.no_stokes else bench_stokes(ctx, R, not args.no_cpu)
 
     # ---- CPU baseline: the oracle's RK4 step on the same workload (all host cores) --------------
     cpu = None
@@ -717,6 +818,7 @@ def main():
             "spmm_4ops": spmm,
             "weights_s": round(t_weights, 4),
             "weights_uneven": weights_c3,
+            "stokes": stokes,
             "knn": knn,
             "tuned_shapes": tuned,
             "device": info,

@@ -34,8 +34,11 @@
 
 namespace {
 
-constexpr int SWEEP_THREADS = 1024;
-constexpr int FACTOR_WAVES = SWEEP_THREADS / 64;
+constexpr int FACTOR_THREADS = 1024;
+constexpr int FACTOR_WAVES = FACTOR_THREADS / 64;
+constexpr int SWEEP_THREADS = 128;      // a level holds a few dozen rows (x3 components)
+constexpr int SWEEP_CH = 32;            // for a row's first 32 entries and their gathers
+constexpr int SWEEP_LDS_LEVELS = 8192;  // level offsets kept in LDS up to this many levels
 constexpr int FACTOR_MAX_WIDTH = 64;  // one lane per stencil entry in the factorization
 constexpr int DOT_THREADS = 256;
 constexpr int DOT_BLOCKS = 256;
@@ -163,7 +166,7 @@ __global__ void constraint_rows_kernel(const double* __restrict__ sums, int N, T
 // lane q holds entry q of the row in LDS; for every L entry k (ascending) the multiplier
 // t = w_k / U_kk is formed and the lanes that hold row k's U entries subtract t * U_kj from the
 // row's entry j where the input holds a nonzero (no fill).
-__global__ void __launch_bounds__(SWEEP_THREADS) ilu0_factor_kernel(const int32_t* __restrict__ ip,
+__global__ void __launch_bounds__(FACTOR_THREADS) ilu0_factor_kernel(const int32_t* __restrict__ ip,
                                                                     const int32_t* __restrict__ col,
                                                                     const int32_t* __restrict__ diag,
                                                                     const double* __restrict__ a_in,
@@ -208,49 +211,108 @@ __global__ void __launch_bounds__(SWEEP_THREADS) ilu0_factor_kernel(const int32_
     }
 }
 
-// Unit-lower sweep v_c[r] -= sum_{q < diag} L_rq v_c[col_q] for the three velocity blocks.
+// One triangular sweep of the factor over the three velocity blocks, level by level. Task t
+// (levels in order) is row rows[t] with its entries packed at [pk_off[t], pk_off[t+1]): the
+// strictly lower entries for the forward sweep, the strictly upper ones (and the diagonal
+// pk_diag[t]) for the backward sweep, columns ascending. Three threads per task, one per
+// velocity component c: v_c[r] -= L_rq v_c[col_q] (or U_rq) entry after entry in column
+// order, then (UPPER) v_c[r] /= U_rr. Only the gathered values depend on the previous level,
+// so a thread loads its next level's task (row, its own value, columns and values of the
+// first SWEEP_CH entries) into a second register set right after issuing this level's
+// gathers: the load overlaps the gathers, the arithmetic and the store.
 template <typename T>
-__global__ void __launch_bounds__(SWEEP_THREADS) ilu0_lower_kernel(const int32_t* __restrict__ ip,
-                                                                   const int32_t* __restrict__ col,
-                                                                   const int32_t* __restrict__ diag,
-                                                                   const T* __restrict__ f,
-                                                                   const int32_t* __restrict__ rows,
-                                                                   const int32_t* __restrict__ off, int n_lev, int N,
-                                                                   T* v) {
-    for (int lev = 0; lev < n_lev; lev++) {
-        const int r0 = off[lev], n3 = 3 * (off[lev + 1] - r0);
-        for (int t = threadIdx.x; t < n3; t += blockDim.x) {
-            const int r = rows[r0 + t / 3];
-            T* vc = v + (size_t)(t % 3) * N;
-            T s = vc[r];
-            for (int q = ip[r]; q < diag[r]; q++) s = add_nc(s, -mul_nc(f[q], vc[col[q]]));
-            vc[r] = s;
+struct SweepTask {
+    int r, a, e;
+    int pc[SWEEP_CH];
+    T pv[SWEEP_CH];
+    T dg;
+    T s;  // the row's own value: written only by this task, so it can be loaded early
+};
+
+template <typename T, bool UPPER>
+__global__ void __launch_bounds__(SWEEP_THREADS) ilu0_sweep_kernel(const int32_t* __restrict__ rows,
+                                                                   const int32_t* __restrict__ off, int n_lev,
+                                                                   const int32_t* __restrict__ pk_off,
+                                                                   const int32_t* __restrict__ pk_col,
+                                                                   const T* __restrict__ pk_val,
+                                                                   const T* __restrict__ pk_diag, int N, T* v) {
+    __shared__ int s_off[SWEEP_LDS_LEVELS + 1];
+    const bool lds_off = n_lev <= SWEEP_LDS_LEVELS;
+    if (lds_off)
+        for (int i = threadIdx.x; i <= n_lev; i += blockDim.x) s_off[i] = off[i];
+    __syncthreads();
+    auto level_start = [&](int l) { return 3 * (lds_off ? s_off[l] : off[l]); };  // in (task, component) units
+    auto fetch = [&](SweepTask<T>& X, int t3) {
+        const int t = t3 / 3;
+        T* vc = v + (size_t)(t3 - 3 * t) * N;
+        X.r = rows[t];
+        X.a = pk_off[t];
+        X.e = pk_off[t + 1];
+#pragma unroll
+        for (int k = 0; k < SWEEP_CH; k++)
+            if (X.a + k < X.e) {
+                X.pc[k] = pk_col[X.a + k];
+                X.pv[k] = pk_val[X.a + k];
+            }
+        if (UPPER) X.dg = pk_diag[t];
+        X.s = vc[X.r];
+        return vc;
+    };
+    auto finish = [&](const SweepTask<T>& Z, T* zv, const T (&g)[SWEEP_CH]) {
+        T s = Z.s;
+#pragma unroll
+        for (int k = 0; k < SWEEP_CH; k++)
+            if (Z.a + k < Z.e) s = add_nc(s, -mul_nc(Z.pv[k], g[k]));
+        for (int q = Z.a + SWEEP_CH; q < Z.e; q++) s = add_nc(s, -mul_nc(pk_val[q], zv[pk_col[q]]));
+        if (UPPER) s = s / Z.dg;
+        zv[Z.r] = s;
+    };
+    // X: this level's first-round task (X.r < 0: none); Y receives the next level's
+    auto run = [&](SweepTask<T>& X, T*& xv, SweepTask<T>& Y, T*& yv, int lev) {
+        T g[SWEEP_CH];
+        if (X.r >= 0) {
+#pragma unroll
+            for (int k = 0; k < SWEEP_CH; k++)
+                if (X.a + k < X.e) g[k] = xv[X.pc[k]];
+        }
+        asm volatile("" ::: "memory");  // the gathers are requested before the next task's loads
+        Y.r = -1;
+        if (lev + 1 < n_lev) {
+            const int t3 = level_start(lev + 1) + (int)threadIdx.x;
+            if (t3 < level_start(lev + 2)) yv = fetch(Y, t3);
+        }
+        if (X.r >= 0) finish(X, xv, g);
+        // levels with more (row, component) tasks than threads: the further rounds load as they go
+        for (int t3 = level_start(lev) + (int)threadIdx.x + (int)blockDim.x; t3 < level_start(lev + 1);
+             t3 += blockDim.x) {
+            T* zv = fetch(X, t3);
+#pragma unroll
+            for (int k = 0; k < SWEEP_CH; k++)
+                if (X.a + k < X.e) g[k] = zv[X.pc[k]];
+            finish(X, zv, g);
         }
         __syncthreads();
+    };
+    SweepTask<T> A, B;
+    A.r = -1;
+    T* xc = v;
+    T* yc = v;
+    if (n_lev > 0) {
+        const int t3 = level_start(0) + (int)threadIdx.x;
+        if (t3 < level_start(1)) xc = fetch(A, t3);
+    }
+    for (int lev = 0; lev < n_lev; lev += 2) {
+        run(A, xc, B, yc, lev);
+        if (lev + 1 < n_lev) run(B, yc, A, xc, lev + 1);
     }
 }
 
-// Upper sweep v_c[r] = (v_c[r] - sum_{q > diag} U_rq v_c[col_q]) / U_rr, levels from the bottom.
+// dst[i] = (T) src[map[i]]
 template <typename T>
-__global__ void __launch_bounds__(SWEEP_THREADS) ilu0_upper_kernel(const int32_t* __restrict__ ip,
-                                                                   const int32_t* __restrict__ col,
-                                                                   const int32_t* __restrict__ diag,
-                                                                   const T* __restrict__ f,
-                                                                   const int32_t* __restrict__ rows,
-                                                                   const int32_t* __restrict__ off, int n_lev, int N,
-                                                                   T* v) {
-    for (int lev = 0; lev < n_lev; lev++) {
-        const int r0 = off[lev], n3 = 3 * (off[lev + 1] - r0);
-        for (int t = threadIdx.x; t < n3; t += blockDim.x) {
-            const int r = rows[r0 + t / 3];
-            T* vc = v + (size_t)(t % 3) * N;
-            T s = vc[r];
-            const int d = diag[r];
-            for (int q = d + 1; q < ip[r + 1]; q++) s = add_nc(s, -mul_nc(f[q], vc[col[q]]));
-            vc[r] = s / f[d];
-        }
-        __syncthreads();
-    }
+__global__ void gather_kernel(const double* __restrict__ src, const int32_t* __restrict__ map, T* __restrict__ dst,
+                              int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = (T)src[map[i]];
 }
 
 __global__ void to_f32_kernel(const double* __restrict__ in, float* __restrict__ out, size_t n) {
@@ -269,12 +331,18 @@ struct rbfx_stokes {
     double eta = 1.0;
     int N = 0, width = 0;
     size_t n_tot = 0;
-    // factor of A_L = -eta L in CSR (sorted columns), FP64 and FP32 copies
+    // factor of A_L = -eta L in CSR (sorted columns)
     int32_t *ip = nullptr, *col = nullptr, *diag = nullptr;
     double *a_in = nullptr, *f64 = nullptr;
-    float* f32 = nullptr;
     int32_t *lrows = nullptr, *loff = nullptr, *urows = nullptr, *uoff = nullptr;
     int n_lev_l = 0, n_lev_u = 0;
+    // the sweeps' packed entries (task order): offsets, columns, source index into the factor,
+    // values in FP64 and FP32, and the diagonal of each backward task
+    int32_t *lpk_off = nullptr, *lpk_col = nullptr, *lpk_src = nullptr;
+    int32_t *upk_off = nullptr, *upk_col = nullptr, *upk_src = nullptr, *upk_dsrc = nullptr;
+    double *lpk64 = nullptr, *upk64 = nullptr, *ud64 = nullptr;
+    float *lpk32 = nullptr, *upk32 = nullptr, *ud32 = nullptr;
+    int l_nnz = 0, u_nnz = 0;
     // GMRES workspace
     int ws_m = -1;
     void* V = nullptr;    // (m + 1) * n_tot values of the run's type (FP64 size)
@@ -289,9 +357,12 @@ struct rbfx_stokes {
 namespace {
 
 void stokes_free(rbfx_stokes* S) {
-    for (void* p : {(void*)S->ip, (void*)S->col, (void*)S->diag, (void*)S->a_in, (void*)S->f64, (void*)S->f32,
+    for (void* p : {(void*)S->ip, (void*)S->col, (void*)S->diag, (void*)S->a_in, (void*)S->f64,
                     (void*)S->lrows, (void*)S->loff, (void*)S->urows, (void*)S->uoff, S->V, S->w, (void*)S->r64,
-                    (void*)S->x32d, (void*)S->partial, (void*)S->hbuf, (void*)S->coef})
+                    (void*)S->x32d, (void*)S->partial, (void*)S->hbuf, (void*)S->coef, (void*)S->lpk_off,
+                    (void*)S->lpk_col, (void*)S->lpk_src, (void*)S->upk_off, (void*)S->upk_col, (void*)S->upk_src,
+                    (void*)S->upk_dsrc, (void*)S->lpk64, (void*)S->upk64, (void*)S->ud64, (void*)S->lpk32,
+                    (void*)S->upk32, (void*)S->ud32})
         if (p) hipFree(p);
 }
 
@@ -331,11 +402,16 @@ template <typename T>
 int precond(rbfx_stokes* S, T* v) {
     rbfx_ctx* ctx = S->ctx;
     hipStream_t st = ctx->stream;
-    const T* f = sizeof(T) == 8 ? (const T*)S->f64 : (const T*)S->f32;
-    ilu0_lower_kernel<T><<<1, SWEEP_THREADS, 0, st>>>(S->ip, S->col, S->diag, f, S->lrows, S->loff, S->n_lev_l, S->N, v);
-    RBFX_LAUNCH_CHECK(ctx, "ilu0_lower_kernel");
-    ilu0_upper_kernel<T><<<1, SWEEP_THREADS, 0, st>>>(S->ip, S->col, S->diag, f, S->urows, S->uoff, S->n_lev_u, S->N, v);
-    RBFX_LAUNCH_CHECK(ctx, "ilu0_upper_kernel");
+    const bool d = sizeof(T) == 8;
+    const T* lv = d ? (const T*)S->lpk64 : (const T*)S->lpk32;
+    const T* uv = d ? (const T*)S->upk64 : (const T*)S->upk32;
+    const T* ud = d ? (const T*)S->ud64 : (const T*)S->ud32;
+    ilu0_sweep_kernel<T, false><<<1, SWEEP_THREADS, 0, st>>>(S->lrows, S->loff, S->n_lev_l, S->lpk_off, S->lpk_col, lv,
+                                                             nullptr, S->N, v);
+    RBFX_LAUNCH_CHECK(ctx, "ilu0_sweep_kernel");
+    ilu0_sweep_kernel<T, true><<<1, SWEEP_THREADS, 0, st>>>(S->urows, S->uoff, S->n_lev_u, S->upk_off, S->upk_col, uv, ud,
+                                                            S->N, v);
+    RBFX_LAUNCH_CHECK(ctx, "ilu0_sweep_kernel");
     return RBFX_OK;
 }
 
@@ -567,6 +643,25 @@ int rbfx_stokes_create(rbfx_ctx* ctx, rbfx_ell* ops64, rbfx_ell* ops32, double e
     std::vector<int32_t> lrows, loff, urows, uoff;
     by_level(lev, nl, lrows, loff);
     by_level(ulev, nu, urows, uoff);
+    // the sweeps' entries packed task by task (level order), columns ascending
+    std::vector<int32_t> lpo(N + 1, 0), lpc, lps, upo(N + 1, 0), upc, ups, uds(N);
+    for (int t = 0; t < N; t++) {
+        const int r = lrows[t];
+        for (int q = ip[r]; q < dg[r]; q++) {
+            lpc.push_back(col[q]);
+            lps.push_back(q);
+        }
+        lpo[t + 1] = (int32_t)lpc.size();
+    }
+    for (int t = 0; t < N; t++) {
+        const int r = urows[t];
+        for (int q = dg[r] + 1; q < ip[r + 1]; q++) {
+            upc.push_back(col[q]);
+            ups.push_back(q);
+        }
+        upo[t + 1] = (int32_t)upc.size();
+        uds[t] = dg[r];
+    }
 
     rbfx_stokes* S = new rbfx_stokes();
     S->ctx = ctx;
@@ -582,20 +677,41 @@ int rbfx_stokes_create(rbfx_ctx* ctx, rbfx_ell* ops64, rbfx_ell* ops32, double e
     if ((rc = upload(ctx, &S->ip, ip)) || (rc = upload(ctx, &S->col, col)) || (rc = upload(ctx, &S->diag, dg)) ||
         (rc = upload(ctx, &S->a_in, a)) || (rc = upload(ctx, &S->lrows, lrows)) || (rc = upload(ctx, &S->loff, loff)) ||
         (rc = upload(ctx, &S->urows, urows)) || (rc = upload(ctx, &S->uoff, uoff)) ||
-        (rc = upload(ctx, &S->f64, a)) || (rc = workspace(S, 1))) {
+        (rc = upload(ctx, &S->f64, a)) || (rc = upload(ctx, &S->lpk_off, lpo)) || (rc = upload(ctx, &S->lpk_col, lpc)) ||
+        (rc = upload(ctx, &S->lpk_src, lps)) || (rc = upload(ctx, &S->upk_off, upo)) ||
+        (rc = upload(ctx, &S->upk_col, upc)) || (rc = upload(ctx, &S->upk_src, ups)) ||
+        (rc = upload(ctx, &S->upk_dsrc, uds)) || (rc = workspace(S, 1))) {
         stokes_free(S);
         delete S;
         return rc;
     }
-    if (hipMalloc((void**)&S->f32, a.size() * sizeof(float)) != hipSuccess) {
+    S->l_nnz = (int)lpc.size();
+    S->u_nnz = (int)upc.size();
+    const size_t ln = std::max<size_t>(1, lpc.size()), un = std::max<size_t>(1, upc.size());
+    if (hipMalloc((void**)&S->lpk64, ln * sizeof(double)) != hipSuccess ||
+        hipMalloc((void**)&S->upk64, un * sizeof(double)) != hipSuccess ||
+        hipMalloc((void**)&S->ud64, N * sizeof(double)) != hipSuccess ||
+        hipMalloc((void**)&S->lpk32, ln * sizeof(float)) != hipSuccess ||
+        hipMalloc((void**)&S->upk32, un * sizeof(float)) != hipSuccess ||
+        hipMalloc((void**)&S->ud32, N * sizeof(float)) != hipSuccess) {
         stokes_free(S);
         delete S;
-        return rbfx_fail(ctx, RBFX_E_NOMEM, "stokes_create: FP32 factor");
+        return rbfx_fail(ctx, RBFX_E_NOMEM, "stokes_create: packed factor entries");
     }
-    ilu0_factor_kernel<<<1, SWEEP_THREADS, 0, ctx->stream>>>(S->ip, S->col, S->diag, S->a_in, S->lrows, S->loff, nl, S->f64);
+    ilu0_factor_kernel<<<1, FACTOR_THREADS, 0, ctx->stream>>>(S->ip, S->col, S->diag, S->a_in, S->lrows, S->loff, nl, S->f64);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) {
-        to_f32_kernel<<<blocks_for(a.size(), 256), 256, 0, ctx->stream>>>(S->f64, S->f32, a.size());
+        hipStream_t st = ctx->stream;
+        if (S->l_nnz) {
+            gather_kernel<double><<<blocks_for(S->l_nnz, 256), 256, 0, st>>>(S->f64, S->lpk_src, S->lpk64, S->l_nnz);
+            gather_kernel<float><<<blocks_for(S->l_nnz, 256), 256, 0, st>>>(S->f64, S->lpk_src, S->lpk32, S->l_nnz);
+        }
+        if (S->u_nnz) {
+            gather_kernel<double><<<blocks_for(S->u_nnz, 256), 256, 0, st>>>(S->f64, S->upk_src, S->upk64, S->u_nnz);
+            gather_kernel<float><<<blocks_for(S->u_nnz, 256), 256, 0, st>>>(S->f64, S->upk_src, S->upk32, S->u_nnz);
+        }
+        gather_kernel<double><<<blocks_for(N, 256), 256, 0, st>>>(S->f64, S->upk_dsrc, S->ud64, N);
+        gather_kernel<float><<<blocks_for(N, 256), 256, 0, st>>>(S->f64, S->upk_dsrc, S->ud32, N);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
