@@ -25,9 +25,7 @@
 // direct solution, returns the least-squares minimiser over the Krylov space after k
 // iterations, and stops at the iteration limit; the assembled matrix applied to constant
 // velocity and pressure gives the closed form (derivative weights annihilate constants).
-//
-// No GPU code implements these: they are the evidence for leaving the implicit solver off
-// the GPU (the method stalls above the 1e-8 target and its sweeps are sequential).
+
 #include "oracle.h"
 
 #include <cmath>
