@@ -312,13 +312,10 @@ def bench_weights(ctx, R, N, cpu, nthreads, degrees=(2, 3), modes=("f64_lu", "f3
     return res
 
 
-def bench_stokes(ctx, R, cpu, level=6, n=31, budget=1000, krylov=50, tol=1e-8, cpu_iters=50):
-    """Steady Stokes on the sphere (the reference's StokesSteadyPDE with GMRES(50) + ILU(0) of the
-    velocity block): icosahedral nodes (level 6: N = 40962), n = 31 kd-tree stencils, eps =
-    0.035 sqrt(N) - 0.1, eta = 1, a seeded body force of spherical harmonics of degree <= 10 on
-    the velocity rows. Times the FP64 solve and the FP32-inner / FP64-refinement solve over the
-    same iteration budget, the operator and preconditioner applications, and the CPU
-    restatement's GMRES on a bounded sample of iterations."""
+def stokes_case(ctx, R, level, n, budget, krylov, tol, cpu, cpu_iters):
+    """One steady-Stokes solve size (see bench_stokes): FP64 and FP32-refinement GMRES over the
+    same step budget, the operator and preconditioner applications, and optionally the CPU
+    restatement's GMRES on its first cpu_iters steps."""
     xyz = R.nodes_icosahedral(level)
     xyz = np.ascontiguousarray(xyz[R.order_morton(xyz)])
     N = xyz.shape[0]
@@ -364,7 +361,7 @@ def bench_stokes(ctx, R, cpu, level=6, n=31, budget=1000, krylov=50, tol=1e-8, c
         res[name] = {"s": round(sec, 3), "iterations": it, "ms_per_iteration": round(1e3 * sec / max(it, 1), 3),
                      "error": err, "converged": bool(err < tol),
                      "true_rel_residual": float(np.linalg.norm(r) / np.linalg.norm(b))}
-        log(f"Stokes {name}: {it} GMRES steps in {sec:.2f}s, preconditioned residual {err:.2e}, "
+        log(f"Stokes N={N} {name}: {it} GMRES steps in {sec:.2f}s, preconditioned residual {err:.2e}, "
             f"residual {res[name]['true_rel_residual']:.2e}")
     res["fp32_ir"]["rel_diff_from_fp64"] = float(np.linalg.norm(xs["fp32_ir"] - xs["fp64"]) / np.linalg.norm(xs["fp64"]))
     # operator and preconditioner applications (HIP events on the context stream)
@@ -403,6 +400,18 @@ def bench_stokes(ctx, R, cpu, level=6, n=31, budget=1000, krylov=50, tol=1e-8, c
     S.close()
     e64.close()
     e32.close()
+    return out
+
+
+def bench_stokes(ctx, R, cpu):
+    """Steady Stokes on the sphere (the reference's StokesSteadyPDE with GMRES(50) + ILU(0) of the
+    velocity block): icosahedral nodes, n = 31 kd-tree stencils, eps = 0.035 sqrt(N) - 0.1,
+    eta = 1, a seeded body force of spherical harmonics of degree <= 10 on the velocity rows.
+    Level 6 (N = 40962, the Stokes workload's size): a budget of 1000 steps (the reference's
+    method stalls there), timed beside the CPU restatement. Level 3 (N = 642), where it
+    converges: both precisions solved to 1e-8 and compared."""
+    out = stokes_case(ctx, R, 6, 31, 1000, 50, 1e-8, cpu, 50)
+    out["converging_case"] = stokes_case(ctx, R, 3, 31, 4000, 50, 1e-8, False, 0)
     log(f"Stokes: {json.dumps(out)}")
     return out
 
