@@ -323,16 +323,19 @@ void RBFFD_HIP::updateWeightsOnGPU(bool) {
     weightsModified = false;
 }
 
-rbfx_ell* RBFFD_HIP::getGPUOperatorSet(DerTypes types) {
+rbfx_ell* RBFFD_HIP::getGPUOperatorSet(DerTypes types) { return getGPUOperatorSet(types, storage); }
+
+rbfx_ell* RBFFD_HIP::getGPUOperatorSet(DerTypes types, int storage_) {
     updateWeightsOnGPU(true);
+    const uint64_t key = (uint64_t)types | ((uint64_t)(uint32_t)storage_ << 32);
     for (auto& p : fused)
-        if (p.first == types) return p.second;
-    rbfx_ell* e = buildEll(types);
-    fused.push_back({types, e});
+        if (p.first == key) return p.second;
+    rbfx_ell* e = buildEll(types, storage_);
+    fused.push_back({key, e});
     return e;
 }
 
-rbfx_ell* RBFFD_HIP::buildEll(DerTypes types) {
+rbfx_ell* RBFFD_HIP::buildEll(DerTypes types, int storage_) {
     size_t S = grid_ref.getStencilsSize();
     std::vector<StencilType>& st = grid_ref.getStencils();
     std::vector<const double*> ptrs;
@@ -355,7 +358,7 @@ rbfx_ell* RBFFD_HIP::buildEll(DerTypes types) {
     }
     rbfx_ell* e = nullptr;
     int rc = rbfx_ell_create(ctx, (int32_t)S, (int32_t)grid_ref.getNodeListSize(), (int32_t)width, d_idx, (int32_t)ptrs.size(),
-                             bits.data(), ptrs.data(), storage, &e);
+                             bits.data(), ptrs.data(), storage_, &e);
     for (double* d : owned) rbfx_dev_free(ctx, d);
     if (rc != RBFX_OK) die(ctx, "rbfx_ell_create", rc);
     // rows (stencil centres) processed along a Morton curve of their nodes: neighbouring
@@ -381,7 +384,7 @@ rbfx_ell* RBFFD_HIP::buildEll(DerTypes types) {
 rbfx_ell* RBFFD_HIP::getGPUWeights(DerType which) {
     updateWeightsOnGPU(true);
     int i = getDerTypeIndx(which);
-    if (!ell_by_type[i]) ell_by_type[i] = buildEll(which);
+    if (!ell_by_type[i]) ell_by_type[i] = buildEll(which, storage);
     return ell_by_type[i];
 }
 

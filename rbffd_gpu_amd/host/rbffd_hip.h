@@ -121,6 +121,7 @@ class RBFFD_HIP : public RBFFD {
     void updateFunctionOnGPU(unsigned int start_indx, unsigned int nb_vals, double* u, bool forceFinish = true);
     rbfx_ell* getGPUWeights(DerType which);          // single-operator set of one type
     rbfx_ell* getGPUOperatorSet(DerTypes types);      // fused set (ascending bit order)
+    rbfx_ell* getGPUOperatorSet(DerTypes types, int storage_);  // the same in a given storage
     const int32_t* getGPUStencils() const { return d_idx; }
     const double* getGPUNodes() const { return d_xyz; }
     unsigned int getStencilPaddedSize() const { return width; }
@@ -136,7 +137,7 @@ class RBFFD_HIP : public RBFFD {
   private:
     void uploadGeometry();
     void ensureWork(unsigned int n);
-    rbfx_ell* buildEll(DerTypes types);  // padded weights of the types -> device operator set
+    rbfx_ell* buildEll(DerTypes types, int storage_);  // padded weights of the types -> device operator set
     rbfx_ctx* ctx = nullptr;
     int storage = RBFX_F64, accum = RBFX_F64, solve_mode = RBFX_SOLVE_F64_LU;
     unsigned int width = 0;
@@ -148,7 +149,7 @@ class RBFFD_HIP : public RBFFD {
     double* d_wpad = nullptr;      // one padded weight array (staging for ELL builds)
     std::vector<int32_t> st_off;
     std::vector<rbfx_ell*> ell_by_type = std::vector<rbfx_ell*>(NUM_DERIVATIVE_TYPES, nullptr);
-    std::vector<std::pair<DerTypes, rbfx_ell*>> fused;
+    std::vector<std::pair<uint64_t, rbfx_ell*>> fused;  // key: types | storage << 32
     void* d_u = nullptr;
     void* d_out = nullptr;
     unsigned int work_n = 0;
