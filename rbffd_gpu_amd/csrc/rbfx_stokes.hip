@@ -36,8 +36,11 @@ namespace {
 
 constexpr int FACTOR_THREADS = 1024;
 constexpr int FACTOR_WAVES = FACTOR_THREADS / 64;
-constexpr int SWEEP_THREADS = 128;      // a level holds a few dozen rows (x3 components)
-constexpr int SWEEP_CH = 32;            // for a row's first 32 entries and their gathers
+constexpr int SWEEP_THREADS = 512;      // a level holds a few dozen rows, up to about a hundred
+constexpr int SWEEP_CH = 32;            // entries of a row on one side of the diagonal, n <= 33
+constexpr int SWEEP_CH_MAX = 64;        // the same, n <= 64
+constexpr int SWEEP_EPL = 2;            // entries per lane: a row takes CH / 2 lanes
+__host__ __device__ constexpr int sweep_slots(int ch) { return SWEEP_THREADS / (ch / SWEEP_EPL); }
 constexpr int SWEEP_LDS_LEVELS = 8192;  // level offsets kept in LDS up to this many levels
 constexpr int FACTOR_MAX_WIDTH = 64;  // one lane per stencil entry in the factorization
 constexpr int DOT_THREADS = 256;
@@ -214,96 +217,126 @@ __global__ void __launch_bounds__(FACTOR_THREADS) ilu0_factor_kernel(const int32
 // One triangular sweep of the factor over the three velocity blocks, level by level. Task t
 // (levels in order) is row rows[t] with its entries packed at [pk_off[t], pk_off[t+1]): the
 // strictly lower entries for the forward sweep, the strictly upper ones (and the diagonal
-// pk_diag[t]) for the backward sweep, columns ascending. Three threads per task, one per
-// velocity component c: v_c[r] -= L_rq v_c[col_q] (or U_rq) entry after entry in column
-// order, then (UPPER) v_c[r] /= U_rr. Only the gathered values depend on the previous level,
-// so a thread loads its next level's task (row, its own value, columns and values of the
-// first SWEEP_CH entries) into a second register set right after issuing this level's
-// gathers: the load overlaps the gathers, the arithmetic and the store.
-template <typename T>
+// pk_diag[t]) for the backward sweep, columns ascending, at most CH of them. A level holds at
+// most sweep_slots(CH) rows (the host splits larger ones) and each row takes a slot of CH / 2
+// lanes: lane l loads entries l and l + CH / 2, gathers the three components at their
+// columns and writes the products L_rq v_c[col_q] to LDS; lanes 0..2 of the slot then subtract
+// them from v_c[r] one after the other in column order (entries past the row's end hold +0,
+// and s - (+0) == s for every s), so each component accumulates exactly as the sequential
+// sweep does, and finally (UPPER) divide by U_rr.
+//
+// Only the gathers depend on the previous level. A task's loads are spread over the two levels
+// before its own: two levels ahead its row and entry range (locate), one level ahead its
+// columns, values, own values and U_rr (load_entries), each issued after the current level's
+// gathers and read only a level later, so the wait on memory a level pays is for its gathers.
+// The level body has no loop (levels are split to fit the slots, CH covers the widest row),
+// which keeps the compiler's memory-counter waits close to exact. Lanes past a row's end read
+// its first entry (one address for all of them; the packed arrays carry CH zero entries past
+// their end); a slot without a row reads nothing but task 0's place, which keeps those loads
+// unconditional, and skips its entries, gathers and store.
+template <typename T, int EPL>
 struct SweepTask {
+    int t;   // task index, -1: none
+    int tt;  // t, or 0 for an empty slot
     int r, a, e;
-    int pc[SWEEP_CH];
-    T pv[SWEEP_CH];
-    T dg;
-    T s;  // the row's own value: written only by this task, so it can be loaded early
+    int pc[EPL];
+    T pv[EPL];
+    T s, dg;  // lanes 0..2: the row's own value of component l (written only by this task), U_rr
 };
 
-template <typename T, bool UPPER>
+template <typename T, bool UPPER, bool LDS_OFF, int CH>
 __global__ void __launch_bounds__(SWEEP_THREADS) ilu0_sweep_kernel(const int32_t* __restrict__ rows,
                                                                    const int32_t* __restrict__ off, int n_lev,
                                                                    const int32_t* __restrict__ pk_off,
                                                                    const int32_t* __restrict__ pk_col,
                                                                    const T* __restrict__ pk_val,
                                                                    const T* __restrict__ pk_diag, int N, T* v) {
-    __shared__ int s_off[SWEEP_LDS_LEVELS + 1];
-    const bool lds_off = n_lev <= SWEEP_LDS_LEVELS;
-    if (lds_off)
+    constexpr int EPL = SWEEP_EPL, SL = CH / EPL, NS = sweep_slots(CH);
+    // a slot's three rows of products, padded so that the twelve summing lanes of a wave fall
+    // in different banks and each row starts on 16 bytes
+    constexpr int PS = (CH * (int)sizeof(T) + 16) / (int)sizeof(T);
+    __shared__ int s_off[LDS_OFF ? SWEEP_LDS_LEVELS + 1 : 1];
+    __shared__ __align__(16) T s_prod[NS][3][PS];
+    if (LDS_OFF)
         for (int i = threadIdx.x; i <= n_lev; i += blockDim.x) s_off[i] = off[i];
     __syncthreads();
-    auto level_start = [&](int l) { return 3 * (lds_off ? s_off[l] : off[l]); };  // in (task, component) units
-    auto fetch = [&](SweepTask<T>& X, int t3) {
-        const int t = t3 / 3;
-        T* vc = v + (size_t)(t3 - 3 * t) * N;
-        X.r = rows[t];
-        X.a = pk_off[t];
-        X.e = pk_off[t + 1];
+    const int slot = (int)threadIdx.x / SL, l = (int)threadIdx.x % SL;
+    const int lc = l < 3 ? l : 2;  // lanes past 2 load a value they do not use
+    T(*P)[PS] = s_prod[slot];
+    auto level_start = [&](int lv) {
+        lv = lv < n_lev ? lv : n_lev;
+        return LDS_OFF ? s_off[lv] : off[lv];
+    };
+    auto wave_sync = [] {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    };
+    auto locate = [&](SweepTask<T, EPL>& X, int lev) {
+        const int t = level_start(lev) + slot;
+        const bool ok = t < level_start(lev + 1);
+        X.t = ok ? t : -1;
+        X.tt = ok ? t : 0;
+        X.r = rows[X.tt];
+        X.a = pk_off[X.tt];
+        X.e = pk_off[X.tt + 1];
+    };
+    auto load_entries = [&](SweepTask<T, EPL>& X) {
+        const int len = X.e - X.a;
+        if (X.t >= 0) {
 #pragma unroll
-        for (int k = 0; k < SWEEP_CH; k++)
-            if (X.a + k < X.e) {
-                X.pc[k] = pk_col[X.a + k];
-                X.pv[k] = pk_val[X.a + k];
+            for (int j = 0; j < EPL; j++) {
+                const int k = l + j * SL;
+                X.pc[j] = pk_col[X.a + (k < len ? k : 0)];
+                X.pv[j] = pk_val[X.a + (k < len ? k : 0)];
             }
-        if (UPPER) X.dg = pk_diag[t];
-        X.s = vc[X.r];
-        return vc;
+            X.s = v[(size_t)lc * N + X.r];
+            if (UPPER) X.dg = pk_diag[X.t];
+        }
     };
-    auto finish = [&](const SweepTask<T>& Z, T* zv, const T (&g)[SWEEP_CH]) {
-        T s = Z.s;
+    // X: level lev's task, loaded; Y: level lev + 1's, located; Z receives lev + 2's
+    auto run = [&](SweepTask<T, EPL>& X, SweepTask<T, EPL>& Y, SweepTask<T, EPL>& Z, int lev) {
+        T g[3][EPL];
+        if (X.t >= 0) {
 #pragma unroll
-        for (int k = 0; k < SWEEP_CH; k++)
-            if (Z.a + k < Z.e) s = add_nc(s, -mul_nc(Z.pv[k], g[k]));
-        for (int q = Z.a + SWEEP_CH; q < Z.e; q++) s = add_nc(s, -mul_nc(pk_val[q], zv[pk_col[q]]));
-        if (UPPER) s = s / Z.dg;
-        zv[Z.r] = s;
-    };
-    // X: this level's first-round task (X.r < 0: none); Y receives the next level's
-    auto run = [&](SweepTask<T>& X, T*& xv, SweepTask<T>& Y, T*& yv, int lev) {
-        T g[SWEEP_CH];
-        if (X.r >= 0) {
+            for (int j = 0; j < EPL; j++)
 #pragma unroll
-            for (int k = 0; k < SWEEP_CH; k++)
-                if (X.a + k < X.e) g[k] = xv[X.pc[k]];
+                for (int c = 0; c < 3; c++) g[c][j] = v[(size_t)c * N + X.pc[j]];
         }
-        asm volatile("" ::: "memory");  // the gathers are requested before the next task's loads
-        Y.r = -1;
-        if (lev + 1 < n_lev) {
-            const int t3 = level_start(lev + 1) + (int)threadIdx.x;
-            if (t3 < level_start(lev + 2)) yv = fetch(Y, t3);
-        }
-        if (X.r >= 0) finish(X, xv, g);
-        // levels with more (row, component) tasks than threads: the further rounds load as they go
-        for (int t3 = level_start(lev) + (int)threadIdx.x + (int)blockDim.x; t3 < level_start(lev + 1);
-             t3 += blockDim.x) {
-            T* zv = fetch(X, t3);
+        asm volatile("" ::: "memory");  // the gathers are requested before the later levels' loads
+        load_entries(Y);
+        locate(Z, lev + 2);
+        const int len = X.t >= 0 ? X.e - X.a : 0;
 #pragma unroll
-            for (int k = 0; k < SWEEP_CH; k++)
-                if (X.a + k < X.e) g[k] = zv[X.pc[k]];
-            finish(X, zv, g);
+        for (int j = 0; j < EPL; j++) {
+            const int k = l + j * SL;
+            const bool in = k < len;
+#pragma unroll
+            for (int c = 0; c < 3; c++) P[c][k] = in ? mul_nc(X.pv[j], g[c][j]) : T(0);
         }
+        wave_sync();
+        // every lane of a slot sums and stores: lanes past 2 repeat lane 2 (same value, same address)
+        T s = X.s;
+#pragma unroll 32
+        for (int k = 0; k < CH; k++) s = add_nc(s, -P[lc][k]);
+        if (UPPER) s = s / X.dg;
+        if (X.t >= 0) v[(size_t)lc * N + X.r] = s;
+        wave_sync();  // the slot's products are read before the next level overwrites them
         __syncthreads();
     };
-    SweepTask<T> A, B;
-    A.r = -1;
-    T* xc = v;
-    T* yc = v;
-    if (n_lev > 0) {
-        const int t3 = level_start(0) + (int)threadIdx.x;
-        if (t3 < level_start(1)) xc = fetch(A, t3);
-    }
-    for (int lev = 0; lev < n_lev; lev += 2) {
-        run(A, xc, B, yc, lev);
-        if (lev + 1 < n_lev) run(B, yc, A, xc, lev + 1);
+    SweepTask<T, EPL> A, B, C;
+    locate(A, 0);
+    load_entries(A);
+    locate(B, 1);
+    // levels past n_lev are empty (level_start clamps), so the loop body needs no test; the
+    // compiler waits for every outstanding access at the loop head, hence the long body
+    for (int lev = 0; lev < n_lev; lev += 6) {
+        run(A, B, C, lev);
+        run(B, C, A, lev + 1);
+        run(C, A, B, lev + 2);
+        run(A, B, C, lev + 3);
+        run(B, C, A, lev + 4);
+        run(C, A, B, lev + 5);
     }
 }
 
@@ -336,6 +369,7 @@ struct rbfx_stokes {
     double *a_in = nullptr, *f64 = nullptr;
     int32_t *lrows = nullptr, *loff = nullptr, *urows = nullptr, *uoff = nullptr;
     int n_lev_l = 0, n_lev_u = 0;
+    int max_side = 0;  // most entries of a row on one side of the diagonal
     // the sweeps' packed entries (task order): offsets, columns, source index into the factor,
     // values in FP64 and FP32, and the diagonal of each backward task
     int32_t *lpk_off = nullptr, *lpk_col = nullptr, *lpk_src = nullptr;
@@ -397,6 +431,25 @@ int apply(rbfx_stokes* S, const T* x, T* y) {
     return RBFX_OK;
 }
 
+template <typename T, bool UPPER>
+void launch_sweep(const rbfx_stokes* S, hipStream_t st, const int32_t* rows, const int32_t* off, int n_lev,
+                  const int32_t* pk_off, const int32_t* pk_col, const T* pk_val, const T* pk_diag, T* v) {
+    const bool lds = n_lev <= SWEEP_LDS_LEVELS;
+    const bool wide = S->max_side > SWEEP_CH;
+    if (lds && !wide)
+        ilu0_sweep_kernel<T, UPPER, true, SWEEP_CH><<<1, SWEEP_THREADS, 0, st>>>(rows, off, n_lev, pk_off, pk_col, pk_val,
+                                                                                pk_diag, S->N, v);
+    else if (lds)
+        ilu0_sweep_kernel<T, UPPER, true, SWEEP_CH_MAX><<<1, SWEEP_THREADS, 0, st>>>(rows, off, n_lev, pk_off, pk_col,
+                                                                                    pk_val, pk_diag, S->N, v);
+    else if (!wide)
+        ilu0_sweep_kernel<T, UPPER, false, SWEEP_CH><<<1, SWEEP_THREADS, 0, st>>>(rows, off, n_lev, pk_off, pk_col,
+                                                                                 pk_val, pk_diag, S->N, v);
+    else
+        ilu0_sweep_kernel<T, UPPER, false, SWEEP_CH_MAX><<<1, SWEEP_THREADS, 0, st>>>(rows, off, n_lev, pk_off, pk_col,
+                                                                                     pk_val, pk_diag, S->N, v);
+}
+
 // v = M^-1 v: both sweeps on the three velocity blocks; the other rows are identity rows.
 template <typename T>
 int precond(rbfx_stokes* S, T* v) {
@@ -406,11 +459,9 @@ int precond(rbfx_stokes* S, T* v) {
     const T* lv = d ? (const T*)S->lpk64 : (const T*)S->lpk32;
     const T* uv = d ? (const T*)S->upk64 : (const T*)S->upk32;
     const T* ud = d ? (const T*)S->ud64 : (const T*)S->ud32;
-    ilu0_sweep_kernel<T, false><<<1, SWEEP_THREADS, 0, st>>>(S->lrows, S->loff, S->n_lev_l, S->lpk_off, S->lpk_col, lv,
-                                                             nullptr, S->N, v);
+    launch_sweep<T, false>(S, st, S->lrows, S->loff, S->n_lev_l, S->lpk_off, S->lpk_col, lv, nullptr, v);
     RBFX_LAUNCH_CHECK(ctx, "ilu0_sweep_kernel");
-    ilu0_sweep_kernel<T, true><<<1, SWEEP_THREADS, 0, st>>>(S->urows, S->uoff, S->n_lev_u, S->upk_off, S->upk_col, uv, ud,
-                                                            S->N, v);
+    launch_sweep<T, true>(S, st, S->urows, S->uoff, S->n_lev_u, S->upk_off, S->upk_col, uv, ud, v);
     RBFX_LAUNCH_CHECK(ctx, "ilu0_sweep_kernel");
     return RBFX_OK;
 }
@@ -640,9 +691,25 @@ int rbfx_stokes_create(rbfx_ctx* ctx, rbfx_ell* ops64, rbfx_ell* ops32, double e
         std::vector<int32_t> pos(off.begin(), off.end() - 1);
         for (int r = 0; r < N; r++) rows[pos[L[r]]++] = r;
     };
+    int max_side = 0;
+    for (int r = 0; r < N; r++) max_side = std::max({max_side, dg[r] - ip[r], ip[r + 1] - dg[r] - 1});
+    // a sweep runs at most sweep_slots rows per step: larger levels become several steps
+    const int slots = sweep_slots(max_side > SWEEP_CH ? SWEEP_CH_MAX : SWEEP_CH);
+    auto split = [&](std::vector<int32_t>& off) {
+        std::vector<int32_t> o(1, 0);
+        for (size_t l = 0; l + 1 < off.size(); l++) {
+            for (int32_t b = off[l] + slots; b < off[l + 1]; b += slots) o.push_back(b);
+            o.push_back(off[l + 1]);
+        }
+        off.swap(o);
+    };
     std::vector<int32_t> lrows, loff, urows, uoff;
     by_level(lev, nl, lrows, loff);
     by_level(ulev, nu, urows, uoff);
+    split(loff);
+    split(uoff);
+    nl = (int)loff.size() - 1;
+    nu = (int)uoff.size() - 1;
     // the sweeps' entries packed task by task (level order), columns ascending
     std::vector<int32_t> lpo(N + 1, 0), lpc, lps, upo(N + 1, 0), upc, ups, uds(N);
     for (int t = 0; t < N; t++) {
@@ -662,6 +729,10 @@ int rbfx_stokes_create(rbfx_ctx* ctx, rbfx_ell* ops64, rbfx_ell* ops32, double e
         upo[t + 1] = (int32_t)upc.size();
         uds[t] = dg[r];
     }
+    // the sweeps read a whole chunk from any task's start: SWEEP_CH_MAX zero entries past the end
+    const int l_nnz = (int)lpc.size(), u_nnz = (int)upc.size();
+    lpc.resize(l_nnz + SWEEP_CH_MAX, 0);
+    upc.resize(u_nnz + SWEEP_CH_MAX, 0);
 
     rbfx_stokes* S = new rbfx_stokes();
     S->ctx = ctx;
@@ -673,6 +744,7 @@ int rbfx_stokes_create(rbfx_ctx* ctx, rbfx_ell* ops64, rbfx_ell* ops32, double e
     S->n_tot = 4 * (size_t)N + 4;
     S->n_lev_l = nl;
     S->n_lev_u = nu;
+    S->max_side = max_side;
     int rc = RBFX_OK;
     if ((rc = upload(ctx, &S->ip, ip)) || (rc = upload(ctx, &S->col, col)) || (rc = upload(ctx, &S->diag, dg)) ||
         (rc = upload(ctx, &S->a_in, a)) || (rc = upload(ctx, &S->lrows, lrows)) || (rc = upload(ctx, &S->loff, loff)) ||
@@ -685,9 +757,9 @@ int rbfx_stokes_create(rbfx_ctx* ctx, rbfx_ell* ops64, rbfx_ell* ops32, double e
         delete S;
         return rc;
     }
-    S->l_nnz = (int)lpc.size();
-    S->u_nnz = (int)upc.size();
-    const size_t ln = std::max<size_t>(1, lpc.size()), un = std::max<size_t>(1, upc.size());
+    S->l_nnz = l_nnz;
+    S->u_nnz = u_nnz;
+    const size_t ln = lpc.size(), un = upc.size();  // with the zero entries past the end
     if (hipMalloc((void**)&S->lpk64, ln * sizeof(double)) != hipSuccess ||
         hipMalloc((void**)&S->upk64, un * sizeof(double)) != hipSuccess ||
         hipMalloc((void**)&S->ud64, N * sizeof(double)) != hipSuccess ||
@@ -698,8 +770,15 @@ int rbfx_stokes_create(rbfx_ctx* ctx, rbfx_ell* ops64, rbfx_ell* ops32, double e
         delete S;
         return rbfx_fail(ctx, RBFX_E_NOMEM, "stokes_create: packed factor entries");
     }
-    ilu0_factor_kernel<<<1, FACTOR_THREADS, 0, ctx->stream>>>(S->ip, S->col, S->diag, S->a_in, S->lrows, S->loff, nl, S->f64);
-    hipError_t e = hipGetLastError();
+    hipError_t e = hipMemsetAsync(S->lpk64, 0, ln * sizeof(double), ctx->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(S->upk64, 0, un * sizeof(double), ctx->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(S->lpk32, 0, ln * sizeof(float), ctx->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(S->upk32, 0, un * sizeof(float), ctx->stream);
+    if (e == hipSuccess) {
+        ilu0_factor_kernel<<<1, FACTOR_THREADS, 0, ctx->stream>>>(S->ip, S->col, S->diag, S->a_in, S->lrows, S->loff, nl,
+                                                                  S->f64);
+        e = hipGetLastError();
+    }
     if (e == hipSuccess) {
         hipStream_t st = ctx->stream;
         if (S->l_nnz) {

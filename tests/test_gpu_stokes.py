@@ -102,6 +102,26 @@ def test_stokes_factor_and_sweeps_bit_identical(ctx, ico3):
     e64.close()
 
 
+@pytest.mark.parametrize("level,n", [(6, 31), (4, 50)])
+def test_stokes_sweeps_bit_identical_large(ctx, level, n):
+    """ico:6 (N = 40962) has levels of more rows than a sweep step has slots (up to 104 rows,
+    split into steps on the host); n = 50 gives rows of up to 49 entries on each side of the
+    diagonal (the sweep's 64-entry variant)."""
+    print(f"case: stokes ILU(0) sweeps, ico:{level}, n={n}", flush=True)
+    xyz, st, w = _case(level, n)
+    N = st.shape[0]
+    S, e64, _ = _stokes(ctx, xyz, st, w)
+    A = O.stokes_assemble(st, w)
+    F = O.ilu0(A, 0, 3 * N)
+    f = S.factor()
+    for c in range(3):
+        assert np.array_equal(f, F.data[F.indptr[c * N]:F.indptr[(c + 1) * N]].reshape(N, n))
+    v = np.random.default_rng(5).standard_normal(A.n)
+    assert np.array_equal(S.precond(v), O.ilu0_apply(F, v))
+    S.close()
+    e64.close()
+
+
 @pytest.mark.parametrize("k", [1, 5])
 def test_stokes_gmres_minimizes_over_krylov_space(ctx, ico2, k):
     """k steps of one cycle return argmin ||M^-1 (b - A x)|| over x in
