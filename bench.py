@@ -379,7 +379,7 @@ def stokes_case(ctx, R, level, n, budget, krylov, tol, cpu, cpu_iters):
     d_y.free()
     d_b.free()
     out = {"N": N, "n": n, "unknowns": S.n, "eps": eps, "krylov": krylov, "tol": tol, "iteration_budget": budget,
-           "sweep_levels": {"lower": S.lower_levels, "upper": S.upper_levels}, "setup_s": round(t_setup, 3),
+           "sweep_steps": {"lower": S.lower_levels, "upper": S.upper_levels}, "setup_s": round(t_setup, 3),
            **res, **times}
     if cpu:
         O, build = cpu_oracle()

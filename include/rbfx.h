@@ -291,7 +291,8 @@ int rbfx_tune_clear(void);
 typedef struct rbfx_stokes rbfx_stokes;
 int rbfx_stokes_create(rbfx_ctx* ctx, rbfx_ell* ops64, rbfx_ell* ops32, double eta, rbfx_stokes** out);
 void rbfx_stokes_destroy(rbfx_stokes* s);
-/* Unknowns (4N + 4) and the number of dependency levels of the two triangular sweeps. */
+/* Unknowns (4N + 4) and the number of steps of the two triangular sweeps: their dependency
+ * levels, each split into steps of at most 32 rows (16 for stencils above 33 nodes). */
 int rbfx_stokes_info(const rbfx_stokes* s, int32_t* n_unknowns, int32_t* lower_levels, int32_t* upper_levels);
 /* d_y = A d_x (device, 4N + 4 doubles each): the reference's prod(LHS, x). */
 int rbfx_stokes_apply(rbfx_stokes* s, const double* d_x, double* d_y);
