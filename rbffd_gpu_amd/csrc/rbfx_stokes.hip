@@ -294,9 +294,11 @@ __global__ void __launch_bounds__(SWEEP_THREADS) ilu0_sweep_kernel(const int32_t
             if (UPPER) X.dg = pk_diag[X.t];
         }
     };
+    // A slot without a row skips its loads and keeps the (defined) values of an earlier task in
+    // the task and gather registers; nothing computed from them is stored.
+    T g[3][EPL] = {};
     // X: level lev's task, loaded; Y: level lev + 1's, located; Z receives lev + 2's
     auto run = [&](SweepTask<T, EPL>& X, SweepTask<T, EPL>& Y, SweepTask<T, EPL>& Z, int lev) {
-        T g[3][EPL];
         if (X.t >= 0) {
 #pragma unroll
             for (int j = 0; j < EPL; j++)
@@ -324,7 +326,7 @@ __global__ void __launch_bounds__(SWEEP_THREADS) ilu0_sweep_kernel(const int32_t
         wave_sync();  // the slot's products are read before the next level overwrites them
         __syncthreads();
     };
-    SweepTask<T, EPL> A, B, C;
+    SweepTask<T, EPL> A{}, B{}, C{};
     locate(A, 0);
     load_entries(A);
     locate(B, 1);
