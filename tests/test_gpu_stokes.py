@@ -122,6 +122,35 @@ def test_stokes_sweeps_bit_identical_large(ctx, level, n):
     e64.close()
 
 
+def test_stokes_sweeps_bit_identical_long_chain(ctx):
+    """Nodes along an arc, each stencil its 5 nearest neighbours: row i depends on row i - 1,
+    so both sweeps have one level per row, more levels (9000) than the sweep keeps in LDS.
+    Random diagonally dominant weights: only the factor and the sweeps are checked."""
+    print("case: stokes ILU(0) sweeps, 9000-level chain", flush=True)
+    N, n = 9000, 5
+    th = np.linspace(0.0, 3.0, N)
+    xyz = np.ascontiguousarray(np.stack([np.cos(th), np.sin(th), np.zeros(N)], axis=1))
+    st = R.knn_kdtree(xyz, n)
+    rng = np.random.default_rng(6)
+    w = rng.uniform(-0.5, 0.5, (4, N, n))
+    w[0] = rng.uniform(0.0, 0.5, (N, n))
+    w[0][:, 0] = -4.0  # column 0 of a kd-tree stencil is the centre
+    ws = [np.ascontiguousarray(w[o]) for o in range(4)]
+    e64 = R.EllOperators(ctx, st, ws, storage=R.F64, types=[R.LSFC, R.XSFC, R.YSFC, R.ZSFC])
+    e64.set_row_order(np.arange(N))
+    S = R.Stokes(ctx, e64)
+    assert S.lower_levels >= 8193 and S.upper_levels >= 8193
+    A = O.stokes_assemble(st, w)
+    F = O.ilu0(A, 0, 3 * N)
+    f = S.factor()
+    for c in range(3):
+        assert np.array_equal(f, F.data[F.indptr[c * N]:F.indptr[(c + 1) * N]].reshape(N, n))
+    v = rng.standard_normal(A.n)
+    assert np.array_equal(S.precond(v), O.ilu0_apply(F, v))
+    S.close()
+    e64.close()
+
+
 @pytest.mark.parametrize("k", [1, 5])
 def test_stokes_gmres_minimizes_over_krylov_space(ctx, ico2, k):
     """k steps of one cycle return argmin ||M^-1 (b - A x)|| over x in
