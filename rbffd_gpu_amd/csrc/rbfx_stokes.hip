@@ -43,6 +43,7 @@ constexpr int SWEEP_EPL = 2;            // entries per lane: a row takes CH / 2 
 __host__ __device__ constexpr int sweep_slots(int ch) { return SWEEP_THREADS / (ch / SWEEP_EPL); }
 constexpr int SWEEP_LDS_LEVELS = 8192;  // level offsets kept in LDS up to this many levels
 constexpr int FACTOR_MAX_WIDTH = 64;  // one lane per stencil entry in the factorization
+constexpr int FACTOR_CHUNK = 8;       // L entries whose U rows a wave loads at once
 constexpr int DOT_THREADS = 256;
 constexpr int DOT_BLOCKS = 256;
 
@@ -168,44 +169,76 @@ __global__ void constraint_rows_kernel(const double* __restrict__ sums, int N, T
 // ILU(0) of A_L (CSR, sorted columns, diagonal present) in dependency levels. One wave per row:
 // lane q holds entry q of the row in LDS; for every L entry k (ascending) the multiplier
 // t = w_k / U_kk is formed and the lanes that hold row k's U entries subtract t * U_kj from the
-// row's entry j where the input holds a nonzero (no fill).
+// row's entry j where the input holds a nonzero (no fill). Nothing but w depends on the order of
+// the L entries, so for FACTOR_CHUNK of them at a time the wave first loads, all at once, each
+// row k's U entries and finds where their columns sit in the row (or that they do not, or that
+// the input holds a zero there); the ordered part then only reads LDS.
 __global__ void __launch_bounds__(FACTOR_THREADS) ilu0_factor_kernel(const int32_t* __restrict__ ip,
                                                                     const int32_t* __restrict__ col,
                                                                     const int32_t* __restrict__ diag,
                                                                     const double* __restrict__ a_in,
                                                                     const int32_t* __restrict__ rows,
                                                                     const int32_t* __restrict__ off, int n_lev,
-                                                                    double* __restrict__ f) {
+                                                                    int nnz, double* __restrict__ f) {
     __shared__ double wv[FACTOR_WAVES][FACTOR_MAX_WIDTH];
     __shared__ int wc[FACTOR_WAVES][FACTOR_MAX_WIDTH];
+    __shared__ double ukk[FACTOR_WAVES][FACTOR_MAX_WIDTH];
+    __shared__ int upos[FACTOR_WAVES][FACTOR_CHUNK][FACTOR_MAX_WIDTH];  // position in the row, -1: none
+    __shared__ double uval[FACTOR_WAVES][FACTOR_CHUNK][FACTOR_MAX_WIDTH];
     const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
     for (int lev = 0; lev < n_lev; lev++) {
         const int r0 = off[lev], r1 = off[lev + 1];
         for (int ri = r0 + wave; ri < r1; ri += FACTOR_WAVES) {
             const int r = rows[ri];
             const int s = ip[r], len = ip[r + 1] - s, dpos = diag[r] - s;
+            bool nz = false;
             if (lane < len) {
                 wv[wave][lane] = a_in[s + lane];
                 wc[wave][lane] = col[s + lane];
+                nz = a_in[s + lane] != 0.0;
             }
+            const uint64_t nzmask = __ballot(nz);  // entries the input holds (updates allowed)
             __builtin_amdgcn_wave_barrier();
-            for (int qq = 0; qq < dpos; qq++) {
-                const int k = wc[wave][qq];
-                const double t = wv[wave][qq] / f[diag[k]];
-                __builtin_amdgcn_wave_barrier();
-                if (lane == 0) wv[wave][qq] = t;
-                const int uk = diag[k] + 1, ue = ip[k + 1];
-                if (lane < ue - uk) {
-                    const int j = col[uk + lane];
-                    int lo = 0, hi = len;  // find j among the row's sorted columns
-                    while (lo < hi) {
-                        const int mid = (lo + hi) >> 1;
-                        if (wc[wave][mid] < j) lo = mid + 1; else hi = mid;
+            for (int q0 = 0; q0 < dpos; q0 += FACTOR_CHUNK) {
+                const int nq = dpos - q0 < FACTOR_CHUNK ? dpos - q0 : FACTOR_CHUNK;
+                if (lane < nq) ukk[wave][lane] = f[diag[wc[wave][q0 + lane]]];
+                // all loads of the chunk first (addresses clamped into the arrays), then the searches
+                int ud[FACTOR_CHUNK], ue[FACTOR_CHUNK], uj[FACTOR_CHUNK];
+                double uu[FACTOR_CHUNK];
+#pragma unroll
+                for (int c = 0; c < FACTOR_CHUNK; c++) ud[c] = diag[wc[wave][q0 + (c < nq ? c : 0)]];
+#pragma unroll
+                for (int c = 0; c < FACTOR_CHUNK; c++) {
+                    const int q = ud[c] + 1 + lane < nnz ? ud[c] + 1 + lane : nnz - 1;
+                    ue[c] = ip[wc[wave][q0 + (c < nq ? c : 0)] + 1];
+                    uj[c] = col[q];
+                    uu[c] = f[q];
+                }
+#pragma unroll
+                for (int c = 0; c < FACTOR_CHUNK; c++) {
+                    int pos = -1;
+                    if (c < nq && lane < ue[c] - ud[c] - 1) {
+                        const int j = uj[c];
+                        int lo = 0, hi = len;  // find j among the row's sorted columns
+                        while (lo < hi) {
+                            const int mid = (lo + hi) >> 1;
+                            if (wc[wave][mid] < j) lo = mid + 1; else hi = mid;
+                        }
+                        if (lo < len && wc[wave][lo] == j && ((nzmask >> lo) & 1)) pos = lo;
                     }
-                    if (lo < len && wc[wave][lo] == j && a_in[s + lo] != 0.0)
-                        wv[wave][lo] = dadd_nc(wv[wave][lo], -dmul_nc(t, f[uk + lane]));
+                    upos[wave][c][lane] = pos;
+                    uval[wave][c][lane] = uu[c];
                 }
                 __builtin_amdgcn_wave_barrier();
+                for (int c = 0; c < nq; c++) {
+                    const int qq = q0 + c;
+                    const double t = wv[wave][qq] / ukk[wave][c];
+                    __builtin_amdgcn_wave_barrier();
+                    if (lane == 0) wv[wave][qq] = t;
+                    const int pos = upos[wave][c][lane];
+                    if (pos >= 0) wv[wave][pos] = dadd_nc(wv[wave][pos], -dmul_nc(t, uval[wave][c][lane]));
+                    __builtin_amdgcn_wave_barrier();
+                }
             }
             if (lane < len) f[s + lane] = wv[wave][lane];
             __builtin_amdgcn_wave_barrier();
@@ -778,7 +811,7 @@ int rbfx_stokes_create(rbfx_ctx* ctx, rbfx_ell* ops64, rbfx_ell* ops32, double e
     if (e == hipSuccess) e = hipMemsetAsync(S->upk32, 0, un * sizeof(float), ctx->stream);
     if (e == hipSuccess) {
         ilu0_factor_kernel<<<1, FACTOR_THREADS, 0, ctx->stream>>>(S->ip, S->col, S->diag, S->a_in, S->lrows, S->loff, nl,
-                                                                  S->f64);
+                                                                  N * W, S->f64);
         e = hipGetLastError();
     }
     if (e == hipSuccess) {
